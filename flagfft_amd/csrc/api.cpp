@@ -1,0 +1,380 @@
+// api.cpp — the extern "C" entry points of include/flagfft.h and include/flagfft_ext.h.
+//
+// Validation order and result codes follow the reference's C API:
+//   flagfftPlanMany            src/exec/c_api_plan.cpp:274-336
+//   descriptor acceptance      src/exec/c_api_internal.cpp:5-120
+//   build_plan error mapping   src/exec/c_api_plan.cpp:43-245
+//   Exec* / SetStream / Destroy / GetPlanDescription  src/exec/c_api_exec.cpp:5-236
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "exec.hpp"
+#include "flagfft.h"
+#include "flagfft_ext.h"
+#include "planner.hpp"
+
+struct flagfftPlan_t {
+  void *impl = nullptr;
+};
+
+namespace flagfft {
+namespace {
+
+struct Desc {
+  int rank = 0;
+  std::vector<int64_t> n, inembed, onembed;
+  int64_t istride = 1, idist = 0, ostride = 1, odist = 0, batch = 1;
+  flagfftType type = FLAGFFT_C2C;
+  Prec prec = Prec::F32;
+  bool real_fwd = false, real_inv = false;
+};
+
+struct PlanImpl {
+  Desc desc;
+  int device = 0;
+  PlanTree tree;
+  std::unique_ptr<Executable> fwd, inv;
+  hipStream_t stream = nullptr;
+  std::string description;
+  std::mutex mu;
+};
+
+std::mutex g_handles_mu;
+std::set<flagfftHandle> g_handles;
+thread_local PlanOptions t_options;
+
+flagfftResult type_metadata(flagfftType type, Desc &d) {
+  switch (type) {
+    case FLAGFFT_C2C: d.prec = Prec::F32; d.real_fwd = false; d.real_inv = false; return FLAGFFT_SUCCESS;
+    case FLAGFFT_Z2Z: d.prec = Prec::F64; d.real_fwd = false; d.real_inv = false; return FLAGFFT_SUCCESS;
+    case FLAGFFT_R2C: d.prec = Prec::F32; d.real_fwd = true; d.real_inv = false; return FLAGFFT_SUCCESS;
+    case FLAGFFT_D2Z: d.prec = Prec::F64; d.real_fwd = true; d.real_inv = false; return FLAGFFT_SUCCESS;
+    case FLAGFFT_C2R: d.prec = Prec::F32; d.real_fwd = false; d.real_inv = true; return FLAGFFT_SUCCESS;
+    case FLAGFFT_Z2D: d.prec = Prec::F64; d.real_fwd = false; d.real_inv = true; return FLAGFFT_SUCCESS;
+  }
+  return FLAGFFT_INVALID_TYPE;
+}
+
+// src/exec/c_api_internal.cpp:60-90
+bool supported_1d(const Desc &d) {
+  if (d.rank != 1 || d.batch <= 0 || d.n.size() != 1 || d.n[0] <= 0) return false;
+  const int64_t n = d.n[0];
+  const int64_t compact = n / 2 + 1, padded = 2 * compact;
+  const int64_t in_len = d.real_inv ? compact : n;
+  const int64_t out_len = d.real_fwd ? compact : n;
+  const bool idist_ok = d.idist == in_len || (d.real_fwd && d.idist == padded);
+  const bool odist_ok = d.odist == out_len || (d.real_inv && d.odist == padded);
+  const bool iemb_ok = d.inembed.size() == 1 && (d.inembed[0] == in_len || (d.real_fwd && d.inembed[0] == padded));
+  const bool oemb_ok = d.onembed.size() == 1 && (d.onembed[0] == out_len || (d.real_inv && d.onembed[0] == padded));
+  return d.istride == 1 && d.ostride == 1 && idist_ok && odist_ok && iemb_ok && oemb_ok;
+}
+
+// src/exec/c_api_internal.cpp:92-120
+bool supported_2d(const Desc &d) {
+  if (d.rank != 2 || d.batch <= 0 || d.n.size() != 2 || d.n[0] <= 0 || d.n[1] <= 0) return false;
+  if (d.istride != 1 || d.ostride != 1) return false;
+  const int64_t n0 = d.n[0], n1 = d.n[1], h1 = n1 / 2 + 1;
+  const int64_t in_logical = d.real_inv ? n0 * h1 : n0 * n1;
+  const int64_t out_logical = d.real_fwd ? n0 * h1 : n0 * n1;
+  return d.idist == in_logical && d.odist == out_logical;
+}
+
+// flagfftPlanMany argument checks up to the descriptor (src/exec/c_api_plan.cpp:285-333).
+flagfftResult make_desc(int rank, const int *n, const int *inembed, int istride, int idist, const int *onembed,
+                        int ostride, int odist, flagfftType type, int batch, Desc &d) {
+  if (rank <= 0 || n == nullptr || batch <= 0) return FLAGFFT_INVALID_VALUE;
+  for (int i = 0; i < rank; ++i)
+    if (n[i] <= 0) return FLAGFFT_INVALID_SIZE;
+  d.rank = rank;
+  d.n.assign(n, n + rank);
+  d.istride = istride;
+  d.idist = idist;
+  d.ostride = ostride;
+  d.odist = odist;
+  d.batch = batch;
+  d.type = type;
+  flagfftResult r = type_metadata(type, d);
+  if (r != FLAGFFT_SUCCESS) return r;
+  if (inembed != nullptr) d.inembed.assign(inembed, inembed + rank);
+  else if (d.real_inv && rank == 1) d.inembed = {d.n[0] / 2 + 1};
+  else if (d.real_inv && rank == 2) d.inembed = {d.n[0], d.n[1] / 2 + 1};
+  else d.inembed = d.n;
+  if (onembed != nullptr) d.onembed.assign(onembed, onembed + rank);
+  else if (d.real_fwd && rank == 1) d.onembed = {d.n[0] / 2 + 1};
+  else if (d.real_fwd && rank == 2) d.onembed = {d.n[0], d.n[1] / 2 + 1};
+  else d.onembed = d.n;
+  if (!supported_1d(d) && !supported_2d(d)) return FLAGFFT_NOT_SUPPORTED;
+  return FLAGFFT_SUCCESS;
+}
+
+PlanDesc plan_desc(const Desc &d) {
+  PlanDesc p;
+  p.rank = d.rank;
+  p.n0 = d.n[0];
+  p.n1 = d.rank == 2 ? d.n[1] : 0;
+  p.type = static_cast<int>(d.type);
+  p.batch = d.batch;
+  p.prec = d.prec;
+  p.real_fwd = d.real_fwd;
+  p.real_inv = d.real_inv;
+  return p;
+}
+
+PlanImpl *checked(flagfftHandle h) {
+  if (h == nullptr) return nullptr;
+  std::lock_guard<std::mutex> lk(g_handles_mu);
+  if (g_handles.find(h) == g_handles.end()) return nullptr;
+  return static_cast<PlanImpl *>(h->impl);
+}
+
+flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return FLAGFFT_INVALID_DEVICE;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return FLAGFFT_INVALID_DEVICE;
+  }
+  try {
+    auto impl = std::make_unique<PlanImpl>();
+    impl->desc = d;
+    impl->device = dev;
+    impl->tree = plan_tree(plan_desc(d), t_options);
+    if (d.real_fwd) {
+      impl->fwd = build_executable(impl->tree, false, t_options, d.idist, d.odist);
+    } else if (d.real_inv) {
+      impl->inv = build_executable(impl->tree, true, t_options, d.idist, d.odist);
+    } else {
+      impl->fwd = build_executable(impl->tree, false, t_options, d.idist, d.odist);
+      impl->inv = build_executable(impl->tree, true, t_options, d.idist, d.odist);
+    }
+    std::ostringstream o;
+    o << impl->tree.describe();
+    o << "\n-- Forward execution --\n" << (impl->fwd ? impl->fwd->describe() : std::string("(not compiled)\n"));
+    if (impl->inv) o << "\n-- Inverse execution --\n" << impl->inv->describe();
+    impl->description = o.str();
+    auto handle = std::make_unique<flagfftPlan_t>();
+    handle->impl = impl.release();
+    flagfftHandle h = handle.release();
+    {
+      std::lock_guard<std::mutex> lk(g_handles_mu);
+      g_handles.insert(h);
+    }
+    *out = h;
+    return FLAGFFT_SUCCESS;
+  } catch (const std::bad_alloc &) {
+    return FLAGFFT_ALLOC_FAILED;
+  } catch (const HipError &e) {
+    std::fprintf(stderr, "[flagfft] plan setup failed: %s\n", e.what());
+    return e.code == hipErrorOutOfMemory ? FLAGFFT_ALLOC_FAILED : FLAGFFT_SETUP_FAILED;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "[flagfft] plan setup failed: %s\n", e.what());
+    return FLAGFFT_SETUP_FAILED;
+  }
+}
+
+flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagfftType expect, int direction,
+                          bool has_direction) {
+  PlanImpl *p = checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  if (in == nullptr || out == nullptr) return FLAGFFT_INVALID_VALUE;
+  if (p->desc.type != expect) return FLAGFFT_INVALID_TYPE;
+  if (has_direction && direction != FLAGFFT_FORWARD && direction != FLAGFFT_INVERSE) return FLAGFFT_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(p->mu);
+  const Executable *ex = nullptr;
+  if (p->desc.real_fwd) ex = p->fwd.get();
+  else if (p->desc.real_inv) ex = p->inv.get();
+  else ex = direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
+  if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
+  int cur = p->device;
+  if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);
+  hipError_t e = ex->run(in, out, p->stream);
+  if (cur != p->device) (void)hipSetDevice(cur);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "[flagfft] exec failed: %s\n", hipGetErrorString(e));
+    return FLAGFFT_EXEC_FAILED;
+  }
+  return FLAGFFT_SUCCESS;
+}
+
+}  // namespace
+}  // namespace flagfft
+
+using flagfft::exec_common;
+
+extern "C" flagfftResult flagfftPlanMany(flagfftHandle *plan, int rank, int *n, int *inembed, int istride, int idist,
+                                         int *onembed, int ostride, int odist, flagfftType type, int batch) {
+  if (plan == nullptr) return FLAGFFT_INVALID_VALUE;
+  *plan = nullptr;
+  flagfft::Desc d;
+  flagfftResult r = flagfft::make_desc(rank, n, inembed, istride, idist, onembed, ostride, odist, type, batch, d);
+  if (r != FLAGFFT_SUCCESS) return r;
+  return flagfft::build_plan(plan, d);
+}
+
+extern "C" flagfftResult flagfftPlan1d(flagfftHandle *plan, int nx, flagfftType type, int batch) {
+  int n[1] = {nx};
+  const bool real_fwd = type == FLAGFFT_R2C || type == FLAGFFT_D2Z;
+  const bool real_inv = type == FLAGFFT_C2R || type == FLAGFFT_Z2D;
+  const int half = nx / 2 + 1;
+  return flagfftPlanMany(plan, 1, n, nullptr, 1, real_inv ? half : nx, nullptr, 1, real_fwd ? half : nx, type, batch);
+}
+
+extern "C" flagfftResult flagfftPlan2d(flagfftHandle *plan, int nx, int ny, flagfftType type) {
+  int n[2] = {nx, ny};
+  const bool real_fwd = type == FLAGFFT_R2C || type == FLAGFFT_D2Z;
+  const bool real_inv = type == FLAGFFT_C2R || type == FLAGFFT_Z2D;
+  const int half = ny / 2 + 1;
+  return flagfftPlanMany(plan, 2, n, nullptr, 1, real_inv ? nx * half : nx * ny, nullptr, 1,
+                         real_fwd ? nx * half : nx * ny, type, 1);
+}
+
+extern "C" flagfftResult flagfftPlan3d(flagfftHandle *plan, int nx, int ny, int nz, flagfftType type) {
+  int n[3] = {nx, ny, nz};
+  return flagfftPlanMany(plan, 3, n, nullptr, 1, nx * ny * nz, nullptr, 1, nx * ny * nz, type, 1);
+}
+
+extern "C" flagfftResult flagfftExecC2C(flagfftHandle plan, flagfftComplex *idata, flagfftComplex *odata,
+                                        int direction) {
+  return exec_common(plan, idata, odata, FLAGFFT_C2C, direction, true);
+}
+
+extern "C" flagfftResult flagfftExecZ2Z(flagfftHandle plan, flagfftDoubleComplex *idata, flagfftDoubleComplex *odata,
+                                        int direction) {
+  return exec_common(plan, idata, odata, FLAGFFT_Z2Z, direction, true);
+}
+
+extern "C" flagfftResult flagfftExecR2C(flagfftHandle plan, flagfftReal *idata, flagfftComplex *odata) {
+  return exec_common(plan, idata, odata, FLAGFFT_R2C, FLAGFFT_FORWARD, false);
+}
+
+extern "C" flagfftResult flagfftExecD2Z(flagfftHandle plan, flagfftDoubleReal *idata, flagfftDoubleComplex *odata) {
+  return exec_common(plan, idata, odata, FLAGFFT_D2Z, FLAGFFT_FORWARD, false);
+}
+
+extern "C" flagfftResult flagfftExecC2R(flagfftHandle plan, flagfftComplex *idata, flagfftReal *odata) {
+  return exec_common(plan, idata, odata, FLAGFFT_C2R, FLAGFFT_INVERSE, false);
+}
+
+extern "C" flagfftResult flagfftExecZ2D(flagfftHandle plan, flagfftDoubleComplex *idata, flagfftDoubleReal *odata) {
+  return exec_common(plan, idata, odata, FLAGFFT_Z2D, FLAGFFT_INVERSE, false);
+}
+
+extern "C" flagfftResult flagfftSetStream(flagfftHandle handle, flagfftStream_t stream) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->stream = static_cast<hipStream_t>(stream);
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" flagfftResult flagfftDestroy(flagfftHandle handle) {
+  if (handle == nullptr) return FLAGFFT_INVALID_PLAN;
+  {
+    std::lock_guard<std::mutex> lk(flagfft::g_handles_mu);
+    auto it = flagfft::g_handles.find(handle);
+    if (it == flagfft::g_handles.end()) return FLAGFFT_INVALID_PLAN;
+    flagfft::g_handles.erase(it);
+  }
+  auto *p = static_cast<flagfft::PlanImpl *>(handle->impl);
+  if (p != nullptr) {
+    { std::lock_guard<std::mutex> lk(p->mu); }
+    delete p;
+  }
+  handle->impl = nullptr;
+  delete handle;
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" const char *flagfftGetPlanDescription(flagfftHandle handle) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return nullptr;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return p->description.c_str();
+}
+
+// ---- extensions -----------------------------------------------------------
+
+extern "C" const char *flagfftxVersion(void) { return "flagfft-gfx950 0.1"; }
+
+extern "C" flagfftResult flagfftxPlanDescribe(int rank, const int *n, flagfftType type, int batch, char *buf,
+                                              size_t len) {
+  if (buf == nullptr || len == 0) return FLAGFFT_INVALID_VALUE;
+  buf[0] = '\0';
+  if (rank <= 0 || n == nullptr || batch <= 0) return FLAGFFT_INVALID_VALUE;
+  for (int i = 0; i < rank; ++i)
+    if (n[i] <= 0) return FLAGFFT_INVALID_SIZE;
+  const bool real_fwd = type == FLAGFFT_R2C || type == FLAGFFT_D2Z;
+  const bool real_inv = type == FLAGFFT_C2R || type == FLAGFFT_Z2D;
+  int64_t in_len = 1, out_len = 1;
+  for (int i = 0; i < rank; ++i) {
+    const int64_t inner = (i == rank - 1) ? (n[i] / 2 + 1) : n[i];
+    in_len *= real_inv ? inner : n[i];
+    out_len *= real_fwd ? inner : n[i];
+  }
+  if (in_len > INT32_MAX || out_len > INT32_MAX) return FLAGFFT_INVALID_SIZE;
+  flagfft::Desc d;
+  flagfftResult r = flagfft::make_desc(rank, n, nullptr, 1, static_cast<int>(in_len), nullptr, 1,
+                                       static_cast<int>(out_len), type, rank == 2 ? 1 : batch, d);
+  if (r != FLAGFFT_SUCCESS) return r;
+  try {
+    flagfft::PlanTree t = flagfft::plan_tree(flagfft::plan_desc(d), flagfft::t_options);
+    std::string s = t.describe();
+    std::strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = '\0';
+    return FLAGFFT_SUCCESS;
+  } catch (const std::exception &) {
+    return FLAGFFT_SETUP_FAILED;
+  }
+}
+
+extern "C" flagfftResult flagfftxPlanStats(flagfftHandle handle, int direction, int *launches, long long *pass_bytes) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  if (launches == nullptr || pass_bytes == nullptr) return FLAGFFT_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(p->mu);
+  const flagfft::Executable *ex = nullptr;
+  if (p->desc.real_fwd) ex = p->fwd.get();
+  else if (p->desc.real_inv) ex = p->inv.get();
+  else ex = direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
+  if (ex == nullptr) return FLAGFFT_INVALID_VALUE;
+  *launches = static_cast<int>(ex->steps.size());
+  long long total = 0;
+  for (const auto &s : ex->steps) total += s.bytes_moved;
+  *pass_bytes = total;
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" flagfftResult flagfftxSetTuning(const flagfftxTuning *t) {
+  flagfft::PlanOptions o;
+  if (t != nullptr) {
+    if (t->leaf_max > 0) {
+      o.leaf_max_f32 = t->leaf_max;
+      o.leaf_max_f64 = t->leaf_max;
+    }
+    if (t->pass_max > 0) {
+      o.pass_max_f32 = t->pass_max;
+      o.pass_max_f64 = t->pass_max;
+    }
+    if (t->units > 0) o.forced_units = t->units;
+    if (t->ept == 8 || t->ept == 16 || t->ept == 32) o.ept = t->ept;
+    else if (t->ept != 0) return FLAGFFT_INVALID_VALUE;
+    if (t->chunk_bytes > 0) o.chunk_bytes = t->chunk_bytes;
+    if (t->npasses < 0 || t->npasses > 4 || t->nradices < 0 || t->nradices > 16) return FLAGFFT_INVALID_VALUE;
+    for (int i = 0; i < t->npasses; ++i) o.forced_passes.push_back(t->passes[i]);
+    for (int i = 0; i < t->nradices; ++i) o.forced_radices.push_back(t->radices[i]);
+  }
+  flagfft::t_options = o;
+  return FLAGFFT_SUCCESS;
+}

@@ -1,0 +1,755 @@
+// exec.cpp — build and run executable plans.
+#include "exec.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <sstream>
+
+namespace flagfft {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw HipError(e, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::DevBuf(size_t b) : bytes(b) {
+  if (b == 0) return;
+  hipError_t e = hipMalloc(&ptr, b);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+    hip_check(e, "hipMalloc");
+  }
+}
+
+DevBuf::~DevBuf() {
+  if (ptr) (void)hipFree(ptr);
+}
+
+void Executable::allocate_workspaces() {
+  ws.clear();
+  for (size_t b : ws_bytes) ws.push_back(std::make_shared<DevBuf>(std::max<size_t>(b, 256)));
+}
+
+hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
+  auto resolve = [&](int id) -> void * {
+    if (id == kBufIn) return const_cast<void *>(in);
+    if (id == kBufOut) return out;
+    return ws[static_cast<size_t>(id - kBufWs0)]->ptr;
+  };
+  for (const Step &s : steps) {
+    void *src = resolve(s.src);
+    void *dst = resolve(s.dst);
+    hipError_t e = hipSuccess;
+    switch (s.kind) {
+      case Step::Pass: {
+        PassArgs a = s.pa;
+        a.in = src;
+        a.out = dst;
+        e = launch_pass(a, s.dbl, s.ept, s.shape, st);
+        break;
+      }
+      case Step::Rader:
+      case Step::Bluestein:
+      case Step::R2CEven:
+      case Step::C2REven: {
+        FusedArgs a = s.fa;
+        a.in = src;
+        a.out = dst;
+        if (s.kind == Step::Rader) e = launch_rader(a, s.dbl, s.ept, s.shape, st);
+        else if (s.kind == Step::Bluestein) e = launch_bluestein(a, s.dbl, s.ept, s.shape, st);
+        else if (s.kind == Step::R2CEven) e = launch_r2c_even(a, s.dbl, s.ept, s.shape, st);
+        else e = launch_c2r_even(a, s.dbl, s.ept, s.shape, st);
+        break;
+      }
+      case Step::RealPost:
+      case Step::RealPre: {
+        RealPassArgs a = s.ra;
+        a.in = src;
+        a.out = dst;
+        e = s.kind == Step::RealPost ? launch_real_post(a, s.dbl, st) : launch_real_pre(a, s.dbl, st);
+        break;
+      }
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+namespace {
+
+const char *buf_name(int id) {
+  static const char *names[] = {"in", "out", "ws0", "ws1", "ws2", "ws3", "ws4", "ws5"};
+  return id >= 0 && id < 8 ? names[id] : "ws?";
+}
+
+int ceil_log2(int64_t v) {
+  int p = 0;
+  while ((int64_t{1} << p) < v) ++p;
+  return p;
+}
+
+// Address stride between consecutive units u, u+1 of a pass.
+int64_t unit_stride(bool col_fast, int64_t tcnt, int64_t ncol, const Geometry &g, int64_t i_stride) {
+  if (col_fast) return ncol > 1 ? g.cs : (tcnt > 1 ? i_stride : g.bs);
+  return tcnt > 1 ? i_stride : (ncol > 1 ? g.cs : g.bs);
+}
+
+bool slot_fast(int64_t elem_stride, int64_t unit_str) { return elem_stride != 1 && unit_str == 1; }
+
+int odd_stride(int ext) { return (ext % 2 == 0) ? ext + 1 : ext; }
+
+}  // namespace
+
+std::string Executable::describe() const {
+  std::ostringstream o;
+  o << "kernels=" << steps.size() << " workspaces=" << ws_bytes.size() << "\n";
+  for (size_t i = 0; i < steps.size(); ++i) {
+    const Step &s = steps[i];
+    o << "  [" << i << "] " << s.label << " " << buf_name(s.src) << "->" << buf_name(s.dst) << " grid=" << s.shape.grid.x
+      << " block=" << s.shape.block.x << " lds=" << s.shape.lds_bytes << "\n";
+  }
+  return o.str();
+}
+
+Builder::Builder(Prec p, const PlanOptions &o, Executable &ex, hipStream_t setup)
+    : prec_(p), opt_(o), ex_(ex), st_(setup) {}
+
+Builder::~Builder() {
+  if (!temps_.empty() || !temp_execs_.empty()) (void)hipStreamSynchronize(st_);
+}
+
+void Builder::finish() {
+  hip_check(hipStreamSynchronize(st_), "plan table generation");
+  temps_.clear();
+  temp_execs_.clear();
+}
+
+int Builder::pick_ws(std::initializer_list<int> avoid, size_t bytes) {
+  for (int id = kBufWs0;; ++id) {
+    if (std::find(avoid.begin(), avoid.end(), id) != avoid.end()) continue;
+    size_t idx = static_cast<size_t>(id - kBufWs0);
+    if (ex_.ws_bytes.size() <= idx) ex_.ws_bytes.resize(idx + 1, 0);
+    ex_.ws_bytes[idx] = std::max(ex_.ws_bytes[idx], bytes);
+    return id;
+  }
+}
+
+const void *Builder::roots(int64_t count, int64_t n, int64_t mult) {
+  std::ostringstream k;
+  k << "roots:" << eb() << ":" << count << ":" << n << ":" << mult;
+  auto it = ex_.tables.find(k.str());
+  if (it != ex_.tables.end()) return it->second->ptr;
+  auto buf = std::make_shared<DevBuf>(static_cast<size_t>(std::max<int64_t>(count, 1)) * eb());
+  hip_check(gen_roots(buf->ptr, dbl(), std::max<int64_t>(count, 1), std::max<int64_t>(n, 1), mult, st_), "gen_roots");
+  ex_.tables[k.str()] = buf;
+  return buf->ptr;
+}
+
+const void *Builder::chirp(int64_t n) {
+  std::string k = "chirp:" + std::to_string(eb()) + ":" + std::to_string(n);
+  auto it = ex_.tables.find(k);
+  if (it != ex_.tables.end()) return it->second->ptr;
+  auto buf = std::make_shared<DevBuf>(static_cast<size_t>(n) * eb());
+  hip_check(gen_chirp(buf->ptr, dbl(), n, st_), "gen_chirp");
+  ex_.tables[k] = buf;
+  return buf->ptr;
+}
+
+const void *Builder::fp64_transform_then_cast(const std::string &key, const std::shared_ptr<DevBuf> &src64,
+                                              int64_t len, double scale) {
+  Core c64 = plan_core(len, Prec::F64, opt_);
+  if (c64.kind != CoreKind::Leaf && c64.kind != CoreKind::MultiPass)
+    throw std::runtime_error("convolution length must be smooth");
+  auto ex64 = std::make_unique<Executable>();
+  {
+    Builder b64(Prec::F64, opt_, *ex64, st_);
+    SeqSet s;
+    s.gi = Geometry{1, 0, len};
+    s.go = Geometry{1, 0, len};
+    Modes m;
+    m.nlog = len;
+    b64.emit_core(c64, s, kBufIn, kBufOut, m);
+    b64.finish();
+  }
+  ex64->allocate_workspaces();
+  auto out64 = std::make_shared<DevBuf>(static_cast<size_t>(len) * 16);
+  hip_check(ex64->run(src64->ptr, out64->ptr, st_), "FP64 convolution kernel transform");
+  auto dst = std::make_shared<DevBuf>(static_cast<size_t>(len) * eb());
+  hip_check(scale_cast(static_cast<const double2 *>(out64->ptr), dst->ptr, dbl(), len, scale, st_), "scale_cast");
+  temps_.push_back(src64);
+  temps_.push_back(out64);
+  temp_execs_.push_back(std::move(ex64));
+  ex_.tables[key] = dst;
+  return dst->ptr;
+}
+
+const void *Builder::bhat_bluestein(int64_t n, int64_t M) {
+  std::string k = "bhatB:" + std::to_string(eb()) + ":" + std::to_string(n) + ":" + std::to_string(M);
+  auto it = ex_.tables.find(k);
+  if (it != ex_.tables.end()) return it->second->ptr;
+  auto b64 = std::make_shared<DevBuf>(static_cast<size_t>(M) * 16);
+  hip_check(gen_bluestein_b(static_cast<double2 *>(b64->ptr), n, M, st_), "gen_bluestein_b");
+  return fp64_transform_then_cast(k, b64, M, 1.0 / static_cast<double>(M));
+}
+
+const void *Builder::rader_idx(int64_t p, int64_t g) {
+  std::string k = "raderidx:" + std::to_string(p);
+  auto it = ex_.tables.find(k);
+  if (it != ex_.tables.end()) return it->second->ptr;
+  std::vector<int32_t> idx(static_cast<size_t>(p - 1));
+  int64_t v = 1;
+  for (int64_t q = 0; q < p - 1; ++q) {
+    idx[static_cast<size_t>(q)] = static_cast<int32_t>(v);
+    v = (v * g) % p;
+  }
+  auto buf = std::make_shared<DevBuf>(idx.size() * sizeof(int32_t));
+  hip_check(hipMemcpyAsync(buf->ptr, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st_),
+            "rader index upload");
+  hip_check(hipStreamSynchronize(st_), "rader index upload");
+  ex_.tables[k] = buf;
+  return buf->ptr;
+}
+
+const void *Builder::bhat_rader(int64_t p, int64_t g) {
+  std::string k = "bhatR:" + std::to_string(eb()) + ":" + std::to_string(p);
+  auto it = ex_.tables.find(k);
+  if (it != ex_.tables.end()) return it->second->ptr;
+  const void *idx = rader_idx(p, g);
+  auto b64 = std::make_shared<DevBuf>(static_cast<size_t>(p - 1) * 16);
+  hip_check(gen_rader_b(static_cast<double2 *>(b64->ptr), static_cast<const int32_t *>(idx), p, st_), "gen_rader_b");
+  return fp64_transform_then_cast(k, b64, p - 1, 1.0 / static_cast<double>(p - 1));
+}
+
+void Builder::fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
+                          int64_t R) {
+  if (radices.size() > static_cast<size_t>(kMaxStages)) throw std::runtime_error("too many radix stages");
+  nst = static_cast<int32_t>(radices.size());
+  int64_t q = 1;
+  for (size_t s = 0; s < radices.size(); ++s) {
+    rad[s] = radices[s];
+    fdq[s] = make_fastdiv(static_cast<uint32_t>(q));
+    twstep[s] = static_cast<int32_t>(R / (q * radices[s]));
+    q *= radices[s];
+  }
+  if (q != R) throw std::runtime_error("radix product mismatch");
+}
+
+namespace {
+
+struct ShapeChoice {
+  int C = 1, tpu = 1, ept = 16;
+  size_t lds = 0;
+};
+
+ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, bool chunked, int64_t nunits) {
+  ShapeChoice best;
+  bool found = false;
+  const bool dbl = eb == 16;
+  const int epts[2] = {dbl ? 8 : 16, dbl ? 16 : 32};
+  for (int ept : epts) {
+    if (o.ept && ept != o.ept) continue;
+    int tpu = static_cast<int>((work_len + ept - 1) / ept);
+    if (tpu < 1) tpu = 1;
+    if (tpu > kMaxBlockThreads) continue;
+    int desired = chunked ? std::max(1, o.chunk_bytes / eb) : std::max(1, o.target_threads / tpu);
+    if (o.forced_units > 0) desired = o.forced_units;
+    if (desired > nunits) desired = static_cast<int>(std::max<int64_t>(nunits, 1));
+    const size_t cap = static_cast<size_t>(chunked ? o.lds_cap_bytes : o.lds_pref_bytes);
+    int C = desired;
+    while (C > 1 && (C * tpu > kMaxBlockThreads || static_cast<size_t>(C) * LS * eb > cap)) --C;
+    if (static_cast<size_t>(C) * LS * eb > static_cast<size_t>(o.lds_cap_bytes)) continue;
+    if (!found || C > best.C) {
+      best.C = C;
+      best.tpu = tpu;
+      best.ept = ept;
+      best.lds = static_cast<size_t>(C) * LS * eb;
+      found = true;
+    }
+  }
+  if (!found) throw std::runtime_error("no launch shape fits the LDS / block limits");
+  return best;
+}
+
+}  // namespace
+
+void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  Step st;
+  st.kind = Step::Pass;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  PassArgs &a = st.pa;
+  const int R = static_cast<int>(c.n);
+  a.gi = s.gi;
+  a.go = s.go;
+  a.R = R;
+  a.tcnt = 1;
+  a.p = 1;
+  a.nseq = c.n;
+  a.nlog = m.nlog ? m.nlog : c.n;
+  a.ncol = static_cast<int32_t>(s.ncol);
+  a.col_fast = s.col_fast ? 1 : 0;
+  a.units = s.nbatch * s.ncol;
+  if (a.units >= (int64_t{1} << 31)) throw std::runtime_error("too many sequences for one launch");
+  fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, c.radices, R);
+  a.tw_small = roots(R, R, 1);
+  a.tw_lo = a.tw_small;
+  a.tw_hi = a.tw_small;
+  a.tw_bits = 0;
+  a.twp_mult = 1;
+  const int LS = odd_stride(lds_pad(R - 1) + 1);
+  const bool lsf = slot_fast(s.gi.es, unit_stride(s.col_fast, 1, s.ncol, s.gi, s.gi.es));
+  const bool ssf = slot_fast(s.go.es, unit_stride(s.col_fast, 1, s.ncol, s.go, s.go.es));
+  ShapeChoice sh = pick_shape(opt_, eb(), R, LS, lsf || ssf, a.units);
+  a.C = sh.C;
+  a.tpu = sh.tpu;
+  a.lds_stride = LS;
+  a.load_slot_fast = lsf;
+  a.store_slot_fast = ssf;
+  a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+  a.fd_R = make_fastdiv(static_cast<uint32_t>(R));
+  a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  a.fd_tcnt = make_fastdiv(1);
+  a.fd_p = make_fastdiv(1);
+  a.src_mode = m.src_mode;
+  a.dst_mode = m.dst_mode;
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  a.chirp = nullptr;
+  a.bhat = nullptr;
+  const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
+  a.nblocks = static_cast<int32_t>(nblocks);
+  a.xcd_remap = 0;
+  st.ept = sh.ept;
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+  st.bytes_moved = a.units * R * eb() * 2;
+  std::ostringstream lab;
+  lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> single-pass R=" << R << " C=" << sh.C
+      << " tpu=" << sh.tpu << " units=" << a.units;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+}
+
+void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, const Modes &first, const Modes &last,
+                             std::initializer_list<int> avoid_extra) {
+  const size_t k = c.passes.size();
+  const int64_t n = c.n;
+  const Geometry gws = s.col_fast ? Geometry{s.ncol, 1, n * s.ncol} : Geometry{1, n, n * s.ncol};
+  const size_t wsbytes = static_cast<size_t>(s.nbatch * s.ncol * n) * eb();
+  std::vector<int> avoid = {src, dst};
+  avoid.insert(avoid.end(), avoid_extra.begin(), avoid_extra.end());
+  auto pick = [&](std::vector<int> av) {
+    for (int id = kBufWs0;; ++id) {
+      if (std::find(av.begin(), av.end(), id) != av.end()) continue;
+      size_t idx = static_cast<size_t>(id - kBufWs0);
+      if (ex_.ws_bytes.size() <= idx) ex_.ws_bytes.resize(idx + 1, 0);
+      ex_.ws_bytes[idx] = std::max(ex_.ws_bytes[idx], wsbytes);
+      return id;
+    }
+  };
+  const int wA = pick(avoid);
+  int wB = -1;
+  if (k > 2) {
+    std::vector<int> av2 = avoid;
+    av2.push_back(wA);
+    wB = pick(av2);
+  }
+  const int bits = (ceil_log2(n) + 1) / 2;
+  const int64_t lo_count = int64_t{1} << bits;
+  const int64_t hi_count = (n + lo_count - 1) / lo_count;
+  const void *tw_lo = roots(lo_count, n, 1);
+  const void *tw_hi = roots(hi_count, n, lo_count);
+  int64_t p = 1;
+  for (size_t j = 0; j < k; ++j) {
+    const int R = static_cast<int>(c.passes[j]);
+    const int64_t tcnt = n / R;
+    Step st;
+    st.kind = Step::Pass;
+    st.dbl = dbl();
+    st.src = (j == 0) ? src : ((j % 2 == 1) ? wA : wB);
+    st.dst = (j + 1 == k) ? dst : (((j + 1) % 2 == 1) ? wA : wB);
+    PassArgs &a = st.pa;
+    a.gi = (j == 0) ? s.gi : gws;
+    a.go = (j + 1 == k) ? s.go : gws;
+    a.R = R;
+    a.tcnt = static_cast<int32_t>(tcnt);
+    a.p = static_cast<int32_t>(p);
+    a.nseq = n;
+    a.ncol = static_cast<int32_t>(s.ncol);
+    a.col_fast = s.col_fast ? 1 : 0;
+    a.units = s.nbatch * s.ncol * tcnt;
+    if (a.units >= (int64_t{1} << 31)) throw std::runtime_error("too many units for one launch");
+    fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, c.pass_radices[j], R);
+    a.tw_small = roots(R, R, 1);
+    a.tw_lo = tw_lo;
+    a.tw_hi = tw_hi;
+    a.tw_bits = bits;
+    a.twp_mult = static_cast<int32_t>(tcnt / p);
+    a.src_mode = (j == 0) ? first.src_mode : kSrcComplex;
+    a.conj_in = (j == 0) ? first.conj_in : 0;
+    a.dst_mode = (j + 1 == k) ? last.dst_mode : kDstComplex;
+    a.conj_out = (j + 1 == k) ? last.conj_out : 0;
+    a.scale = (j + 1 == k) ? last.scale : 1.0;
+    const int64_t nl = (j == 0) ? first.nlog : last.nlog;
+    a.nlog = nl ? nl : n;
+    a.chirp = nullptr;
+    a.bhat = nullptr;  // the pointwise pass of a Bluestein chain gets its table from emit_bluestein
+    if ((a.src_mode & kSrcChirpFlag) || (a.dst_mode & kDstChirpFlag)) a.chirp = chirp(a.nlog);
+    const int LS = odd_stride(lds_pad(R - 1) + 1);
+    const int64_t in_i_stride = a.gi.es;
+    const int64_t out_i_stride = (p > 1) ? a.go.es : static_cast<int64_t>(R) * a.go.es;
+    const bool lsf = slot_fast(tcnt * a.gi.es, unit_stride(s.col_fast, tcnt, s.ncol, a.gi, in_i_stride));
+    const bool ssf = slot_fast(p * a.go.es, unit_stride(s.col_fast, tcnt, s.ncol, a.go, out_i_stride));
+    ShapeChoice sh = pick_shape(opt_, eb(), R, LS, true, a.units);
+    a.C = sh.C;
+    a.tpu = sh.tpu;
+    a.lds_stride = LS;
+    a.load_slot_fast = lsf;
+    a.store_slot_fast = ssf;
+    a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+    a.fd_R = make_fastdiv(static_cast<uint32_t>(R));
+    a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+    a.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
+    a.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+    const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
+    a.nblocks = static_cast<int32_t>(nblocks);
+    a.xcd_remap = opt_.xcd_remap;
+    st.ept = sh.ept;
+    st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+    st.bytes_moved = a.units * R * eb() * 2;
+    std::ostringstream lab;
+    lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> stockham-pass " << j << "/" << k << " R=" << R
+        << " p=" << p << " tcnt=" << tcnt << " C=" << sh.C << " tpu=" << sh.tpu;
+    st.label = lab.str();
+    ex_.steps.push_back(st);
+    p *= R;
+  }
+}
+
+void Builder::emit_rader(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  Step st;
+  st.kind = Step::Rader;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  FusedArgs &a = st.fa;
+  const int p = static_cast<int>(c.n), mm = static_cast<int>(c.m);
+  a.gi = s.gi;
+  a.go = s.go;
+  a.nseqs = s.nbatch * s.ncol;
+  a.ncol = static_cast<int32_t>(s.ncol);
+  a.n = p;
+  a.m = mm;
+  fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, c.radices, mm);
+  a.tw_small = roots(mm, mm, 1);
+  a.aux = rader_idx(p, c.root);
+  a.bhat = bhat_rader(p, c.root);
+  a.chirp = nullptr;
+  const int LS = odd_stride(lds_pad(p) + 1 + lds_pad(mm) + 1 + 1);
+  const int64_t seq_stride_in = s.ncol > 1 ? s.gi.cs : s.gi.bs;
+  const int64_t seq_stride_out = s.ncol > 1 ? s.go.cs : s.go.bs;
+  const bool lsf = slot_fast(s.gi.es, seq_stride_in);
+  const bool ssf = slot_fast(s.go.es, seq_stride_out);
+  ShapeChoice sh = pick_shape(opt_, eb(), mm, LS, lsf || ssf, a.nseqs);
+  a.C = sh.C;
+  a.tpu = sh.tpu;
+  a.lds_stride = LS;
+  a.load_slot_fast = lsf;
+  a.store_slot_fast = ssf;
+  a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+  a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  a.fd_m = make_fastdiv(static_cast<uint32_t>(mm));
+  a.lext = p;
+  a.sext = p;
+  a.fd_lext = make_fastdiv(static_cast<uint32_t>(p));
+  a.fd_sext = make_fastdiv(static_cast<uint32_t>(p));
+  a.src_mode = m.src_mode & 3;
+  a.dst_mode = m.dst_mode & 3;
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
+  st.ept = sh.ept;
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+  st.bytes_moved = a.nseqs * p * eb() * 2;
+  std::ostringstream lab;
+  lab << "k_rader<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> p=" << p << " m=" << mm << " C=" << sh.C
+      << " tpu=" << sh.tpu;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+}
+
+void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  const int64_t n = c.n, M = c.m;
+  if (c.conv) {
+    // two multi-pass length-M transforms; chirp / pointwise / finalize fused into their end passes
+    const Geometry gws = s.col_fast ? Geometry{s.ncol, 1, M * s.ncol} : Geometry{1, M, M * s.ncol};
+    const int wP = pick_ws({src, dst}, static_cast<size_t>(s.nbatch * s.ncol * M) * eb());
+    SeqSet sa = s;
+    sa.go = gws;
+    SeqSet sb = s;
+    sb.gi = gws;
+    Modes first = m;
+    first.src_mode = (m.src_mode & 3) | kSrcChirpFlag;
+    first.nlog = n;
+    Modes mid;
+    mid.dst_mode = kDstPointwiseFlag;
+    mid.nlog = n;
+    emit_multipass(*c.conv, sa, src, wP, first, mid, {dst});
+    ex_.steps.back().pa.bhat = bhat_bluestein(n, M);
+    Modes mfirst;
+    mfirst.nlog = n;
+    Modes fin = m;
+    fin.dst_mode = (m.dst_mode & 3) | kDstChirpFlag;
+    fin.nlog = n;
+    emit_multipass(*c.conv, sb, wP, dst, mfirst, fin, {src});
+    return;
+  }
+  Step st;
+  st.kind = Step::Bluestein;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  FusedArgs &a = st.fa;
+  a.gi = s.gi;
+  a.go = s.go;
+  a.nseqs = s.nbatch * s.ncol;
+  a.ncol = static_cast<int32_t>(s.ncol);
+  a.n = static_cast<int32_t>(n);
+  a.m = static_cast<int32_t>(M);
+  fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, c.radices, M);
+  a.tw_small = roots(M, M, 1);
+  a.chirp = chirp(n);
+  a.bhat = bhat_bluestein(n, M);
+  a.aux = nullptr;
+  const int LS = odd_stride(lds_pad(static_cast<int>(M) - 1) + 1);
+  const int64_t seq_stride_in = s.ncol > 1 ? s.gi.cs : s.gi.bs;
+  const int64_t seq_stride_out = s.ncol > 1 ? s.go.cs : s.go.bs;
+  const bool lsf = slot_fast(s.gi.es, seq_stride_in);
+  const bool ssf = slot_fast(s.go.es, seq_stride_out);
+  ShapeChoice sh = pick_shape(opt_, eb(), M, LS, lsf || ssf, a.nseqs);
+  a.C = sh.C;
+  a.tpu = sh.tpu;
+  a.lds_stride = LS;
+  a.load_slot_fast = lsf;
+  a.store_slot_fast = ssf;
+  a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+  a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  a.fd_m = make_fastdiv(static_cast<uint32_t>(M));
+  a.lext = static_cast<int32_t>(n);
+  a.sext = static_cast<int32_t>(n);
+  a.fd_lext = make_fastdiv(static_cast<uint32_t>(n));
+  a.fd_sext = make_fastdiv(static_cast<uint32_t>(n));
+  a.src_mode = m.src_mode & 3;
+  a.dst_mode = m.dst_mode & 3;
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
+  st.ept = sh.ept;
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+  st.bytes_moved = a.nseqs * n * eb() * 2;
+  std::ostringstream lab;
+  lab << "k_bluestein<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> n=" << n << " M=" << M << " C=" << sh.C
+      << " tpu=" << sh.tpu;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+}
+
+void Builder::emit_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  switch (c.kind) {
+    case CoreKind::Leaf: emit_leaf(c, s, src, dst, m); break;
+    case CoreKind::MultiPass: emit_multipass(c, s, src, dst, m, m, {}); break;
+    case CoreKind::Rader: emit_rader(c, s, src, dst, m); break;
+    case CoreKind::Bluestein: emit_bluestein(c, s, src, dst, m); break;
+  }
+}
+
+void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, bool inverse) {
+  const int64_t n = pl.n;
+  if (pl.real_form == RealForm::None) {
+    Modes m;
+    m.conj_in = inverse;
+    m.conj_out = inverse;
+    m.nlog = n;
+    emit_core(pl.core, s, src, dst, m);
+    return;
+  }
+  if (pl.real_form == RealForm::Full) {
+    Modes m;
+    m.nlog = n;
+    if (pl.real_fwd) {
+      m.src_mode = kSrcReal;
+      m.dst_mode = kDstHalf;
+    } else {
+      m.src_mode = kSrcHermitian;
+      m.dst_mode = kDstReal;
+      m.conj_in = true;
+      m.conj_out = true;
+    }
+    emit_core(pl.core, s, src, dst, m);
+    return;
+  }
+  const int64_t h = n / 2;
+  if (pl.real_form == RealForm::EvenFused) {
+    Step st;
+    st.kind = pl.real_fwd ? Step::R2CEven : Step::C2REven;
+    st.dbl = dbl();
+    st.src = src;
+    st.dst = dst;
+    FusedArgs &a = st.fa;
+    a.gi = s.gi;
+    a.go = s.go;
+    a.nseqs = s.nbatch * s.ncol;
+    a.ncol = static_cast<int32_t>(s.ncol);
+    a.n = static_cast<int32_t>(n);
+    a.m = static_cast<int32_t>(h);
+    fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, pl.core.radices, h);
+    a.tw_small = roots(h, h, 1);
+    a.aux = roots(h + 1, n, 1);
+    a.bhat = nullptr;
+    a.chirp = nullptr;
+    const int LS = odd_stride(lds_pad(static_cast<int>(h) - 1) + 1);
+    ShapeChoice sh = pick_shape(opt_, eb(), h, LS, false, a.nseqs);
+    a.C = sh.C;
+    a.tpu = sh.tpu;
+    a.lds_stride = LS;
+    a.load_slot_fast = 0;
+    a.store_slot_fast = 0;
+    a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+    a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+    a.fd_m = make_fastdiv(static_cast<uint32_t>(h));
+    a.lext = static_cast<int32_t>(h);
+    a.sext = static_cast<int32_t>(pl.real_fwd ? h + 1 : h);
+    a.fd_lext = make_fastdiv(static_cast<uint32_t>(a.lext));
+    a.fd_sext = make_fastdiv(static_cast<uint32_t>(a.sext));
+    a.src_mode = 0;
+    a.dst_mode = 0;
+    a.conj_in = 0;
+    a.conj_out = 0;
+    a.scale = 1.0;
+    const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
+    st.ept = sh.ept;
+    st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+    st.bytes_moved = a.nseqs * (n * eb() / 2 + (h + 1) * eb());
+    std::ostringstream lab;
+    lab << (pl.real_fwd ? "k_r2c_even<" : "k_c2r_even<") << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> n=" << n
+        << " half=" << h << " C=" << sh.C << " tpu=" << sh.tpu;
+    st.label = lab.str();
+    ex_.steps.push_back(st);
+    return;
+  }
+  // EvenMulti: n/2-point multi-pass chain plus a split-spectrum pass
+  const Geometry gws{1, h, h * s.ncol};
+  const size_t wbytes = static_cast<size_t>(s.nbatch * s.ncol * h) * eb();
+  const void *tw_half = roots(h + 1, n, 1);
+  if (pl.real_fwd) {
+    const int W = pick_ws({src, dst}, wbytes);
+    SeqSet sc = s;
+    sc.gi = Geometry{1, s.gi.cs / 2, s.gi.bs / 2};
+    sc.go = gws;
+    Modes m;
+    emit_multipass(pl.core, sc, src, W, m, m, {dst});
+    Step st;
+    st.kind = Step::RealPost;
+    st.dbl = dbl();
+    st.src = W;
+    st.dst = dst;
+    st.ra.gi = gws;
+    st.ra.go = s.go;
+    st.ra.nseqs = s.nbatch * s.ncol;
+    st.ra.ncol = static_cast<int32_t>(s.ncol);
+    st.ra.n = static_cast<int32_t>(n);
+    st.ra.tw_half = tw_half;
+    st.label = "k_real_post n=" + std::to_string(n);
+    ex_.steps.push_back(st);
+  } else {
+    const int W = pick_ws({src, dst}, wbytes);
+    Step st;
+    st.kind = Step::RealPre;
+    st.dbl = dbl();
+    st.src = src;
+    st.dst = W;
+    st.ra.gi = s.gi;
+    st.ra.go = gws;
+    st.ra.nseqs = s.nbatch * s.ncol;
+    st.ra.ncol = static_cast<int32_t>(s.ncol);
+    st.ra.n = static_cast<int32_t>(n);
+    st.ra.tw_half = tw_half;
+    st.label = "k_real_pre n=" + std::to_string(n);
+    ex_.steps.push_back(st);
+    SeqSet sc = s;
+    sc.gi = gws;
+    sc.go = Geometry{1, s.go.cs / 2, s.go.bs / 2};
+    Modes m;
+    Modes last;
+    last.conj_out = true;
+    emit_multipass(pl.core, sc, W, dst, m, last, {src});
+  }
+}
+
+std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o, int64_t idist,
+                                             int64_t odist) {
+  auto ex = std::make_unique<Executable>();
+  Builder b(t.desc.prec, o, *ex, nullptr);
+  const int64_t batch = t.desc.batch;
+  if (t.desc.rank == 1) {
+    SeqSet s;
+    s.nbatch = batch;
+    s.ncol = 1;
+    s.gi = Geometry{1, 0, idist};
+    s.go = Geometry{1, 0, odist};
+    b.emit_plan1d(t.rows, s, kBufIn, kBufOut, inverse);
+  } else {
+    const int64_t n0 = t.desc.n0, n1 = t.desc.n1, h1 = n1 / 2 + 1;
+    if (!t.desc.real_fwd && !t.desc.real_inv) {
+      SeqSet r;
+      r.nbatch = batch * n0;
+      r.gi = Geometry{1, 0, n1};
+      r.go = Geometry{1, 0, n1};
+      b.emit_plan1d(t.rows, r, kBufIn, kBufOut, inverse);
+      SeqSet c;
+      c.nbatch = batch;
+      c.ncol = n1;
+      c.gi = Geometry{n1, 1, n0 * n1};
+      c.go = c.gi;
+      c.col_fast = true;
+      b.emit_plan1d(t.cols, c, kBufOut, kBufOut, inverse);
+    } else if (t.desc.real_fwd) {
+      SeqSet r;
+      r.nbatch = batch * n0;
+      r.gi = Geometry{1, 0, n1};
+      r.go = Geometry{1, 0, h1};
+      b.emit_plan1d(t.rows, r, kBufIn, kBufOut, false);
+      SeqSet c;
+      c.nbatch = batch;
+      c.ncol = h1;
+      c.gi = Geometry{h1, 1, n0 * h1};
+      c.go = c.gi;
+      c.col_fast = true;
+      b.emit_plan1d(t.cols, c, kBufOut, kBufOut, false);
+    } else {
+      const int W2 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
+      SeqSet c;
+      c.nbatch = batch;
+      c.ncol = h1;
+      c.gi = Geometry{h1, 1, n0 * h1};
+      c.go = c.gi;
+      c.col_fast = true;
+      b.emit_plan1d(t.cols, c, kBufIn, W2, true);
+      SeqSet r;
+      r.nbatch = batch * n0;
+      r.gi = Geometry{1, 0, h1};
+      r.go = Geometry{1, 0, n1};
+      b.emit_plan1d(t.rows, r, W2, kBufOut, true);
+    }
+  }
+  b.finish();
+  ex->allocate_workspaces();
+  return ex;
+}
+
+}  // namespace flagfft

@@ -1,0 +1,122 @@
+// exec.hpp — executable plans: ordered kernel launches over user buffers,
+// plan-owned workspaces and FP64-generated device tables.
+//
+// Plays the role of the reference's CompiledRaw* node tree
+// (src/exec/raw_nodes.cpp:41-852) with ahead-of-time compiled HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fft_args.hpp"
+#include "kernels.hpp"
+#include "planner.hpp"
+
+namespace flagfft {
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+
+// Device allocation owned by a plan.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  explicit DevBuf(size_t b);
+  ~DevBuf();
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+enum BufId : int { kBufIn = 0, kBufOut = 1, kBufWs0 = 2 };
+
+struct Step {
+  enum Kind { Pass, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre } kind = Pass;
+  bool dbl = false;
+  int ept = 16;
+  int src = kBufIn, dst = kBufOut;
+  PassArgs pa{};
+  FusedArgs fa{};
+  RealPassArgs ra{};
+  LaunchShape shape{};
+  int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)
+  std::string label;
+};
+
+class Executable {
+ public:
+  std::vector<Step> steps;
+  std::vector<size_t> ws_bytes;                 // per workspace id (kBufWs0 + i)
+  std::vector<std::shared_ptr<DevBuf>> ws;
+  std::map<std::string, std::shared_ptr<DevBuf>> tables;
+
+  // Enqueue every step on `st`; in/out are device pointers.
+  hipError_t run(const void *in, void *out, hipStream_t st) const;
+  std::string describe() const;
+  void allocate_workspaces();
+};
+
+// Geometry of a batch of equal-length sequences in one buffer.
+struct SeqSet {
+  int64_t nbatch = 1, ncol = 1;
+  Geometry gi{1, 0, 0}, go{1, 0, 0};
+  bool col_fast = false;
+};
+
+struct Modes {
+  int src_mode = kSrcComplex, dst_mode = kDstComplex;
+  bool conj_in = false, conj_out = false;
+  int64_t nlog = 0;
+  double scale = 1.0;
+};
+
+class Builder {
+ public:
+  Builder(Prec p, const PlanOptions &o, Executable &ex, hipStream_t setup);
+  ~Builder();
+
+  void emit_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, bool inverse);
+  int pick_ws(std::initializer_list<int> avoid, size_t bytes);
+  void finish();  // wait for table generation, drop temporaries
+
+ private:
+  Prec prec_;
+  PlanOptions opt_;
+  Executable &ex_;
+  hipStream_t st_;
+  std::vector<std::shared_ptr<DevBuf>> temps_;
+  std::vector<std::unique_ptr<Executable>> temp_execs_;
+
+  bool dbl() const { return prec_ == Prec::F64; }
+  int eb() const { return elem_bytes(prec_); }
+  const void *roots(int64_t count, int64_t n, int64_t mult);
+  const void *chirp(int64_t n);
+  const void *bhat_bluestein(int64_t n, int64_t M);
+  const void *rader_idx(int64_t p, int64_t g);
+  const void *bhat_rader(int64_t p, int64_t g);
+  const void *fp64_transform_then_cast(const std::string &key, const std::shared_ptr<DevBuf> &src64, int64_t len,
+                                       double scale);
+
+  void emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void emit_multipass(const Core &c, const SeqSet &s, int src, int dst, const Modes &first, const Modes &last,
+                      std::initializer_list<int> avoid_extra);
+  void emit_rader(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
+                   int64_t R);
+};
+
+// Whole-plan builders (forward / inverse executables of a PlanTree).
+std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o,
+                                             int64_t idist, int64_t odist);
+
+}  // namespace flagfft

@@ -1,0 +1,131 @@
+// fft_args.hpp — kernel argument blocks shared by the host executor and the
+// gfx950 kernels.  Plain structs, passed by value in the kernarg segment.
+#pragma once
+
+#include <cstdint>
+
+namespace flagfft {
+
+constexpr int kMaxStages = 16;       // small-radix stages inside one LDS transform
+constexpr int kLdsBytesMax = 160 * 1024;  // gfx950 LDS per CU
+
+// Unsigned 32-bit division by a run-time invariant (Granlund-Montgomery):
+// q = (umulhi(n, mul) + n) >> shift, valid for n < 2^31.
+struct FastDiv {
+  uint32_t d;
+  uint32_t mul;
+  uint32_t shift;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f{d, 0, 0};
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  f.shift = l;
+  // mul = floor(2^32 * (2^l - d) / d) + 1
+  f.mul = static_cast<uint32_t>(((static_cast<uint64_t>(1) << 32) * ((static_cast<uint64_t>(1) << l) - d)) / d + 1);
+  return f;
+}
+
+// Source of logical element e of an input sequence (first pass / fused kernels).
+enum SrcMode : int32_t {
+  kSrcComplex = 0,   // complex at in[b*bs + c*cs + e*es]
+  kSrcReal = 1,      // real at in[...], imaginary part 0
+  kSrcHermitian = 2, // n/2+1 complex bins expanded to n (Im of DC/Nyquist dropped)
+  kSrcChirpFlag = 4, // Bluestein: e < n -> base(e) * chirp[e], n <= e < M -> 0
+};
+
+// Sink of logical output element e (last pass / fused kernels).
+enum DstMode : int32_t {
+  kDstComplex = 0,   // complex at out[...]
+  kDstHalf = 1,      // complex, only e <= n/2
+  kDstReal = 2,      // real part only
+  kDstPointwiseFlag = 8,  // Bluestein middle: v = conj(v * bhat[e]), complex store
+  kDstChirpFlag = 16,     // Bluestein end: e < n only, v = conj(v) * chirp[e] * scale, then base sink
+};
+
+struct Geometry {
+  // element e of sequence (b, c) lives at base + b*bs + c*cs + e*es
+  // (units of complex elements, or of reals for real sources / sinks)
+  int64_t es, cs, bs;
+};
+
+// One Stockham super-stage (or a whole transform when tcnt == 1) over a batch
+// of sequences.  A "unit" is one length-R DFT: unit u -> (b, c, i), i < tcnt.
+struct PassArgs {
+  const void *in;
+  void *out;
+  Geometry gi, go;
+  int64_t units;       // nbatch * ncol * tcnt
+  int64_t nseq;        // logical sequence length (R * tcnt) — Bluestein: conv length M
+  int64_t nlog;        // user length n (real / Hermitian / chirp modes)
+  int32_t R;           // DFT length computed in LDS by one unit
+  int32_t tcnt;        // units per sequence = nseq / R
+  int32_t p;           // prior product (1 on the first pass)
+  int32_t ncol;
+  int32_t col_fast;    // unit order: 1 -> c fastest, 0 -> i fastest
+  int32_t C;           // units per workgroup
+  int32_t tpu;         // threads per unit
+  int32_t lds_stride;  // padded complex slots per unit in LDS
+  int32_t load_slot_fast, store_slot_fast;  // coalescing order of the staged global accesses
+  int32_t nst;
+  int32_t rad[kMaxStages];
+  FastDiv fd_q[kMaxStages];  // divide by the prior product q_s of each LDS stage
+  FastDiv fd_C, fd_R, fd_ncol, fd_tcnt, fd_p;
+  int32_t twq_step[kMaxStages];  // R / (q_s * rad_s)
+  const void *tw_small;  // w_R^j, j < R
+  const void *tw_lo;     // w_nseq^l, l < 2^tw_bits
+  const void *tw_hi;     // w_nseq^(h 2^tw_bits)
+  int32_t tw_bits;
+  int32_t twp_mult;      // tcnt / p: super-stage twiddle index = e * k * twp_mult
+  int32_t src_mode, dst_mode;
+  int32_t conj_in, conj_out;
+  double scale;          // applied on the final sink (1 unless stated)
+  const void *chirp;     // Bluestein chirp (nlog entries)
+  const void *bhat;      // Bluestein / Rader FFT of the convolution kernel
+  int32_t xcd_remap;     // map consecutive tiles to one XCD
+  int32_t nblocks;
+};
+
+// Fused single-kernel transforms that keep a whole sequence in LDS.
+struct FusedArgs {
+  const void *in;
+  void *out;
+  Geometry gi, go;
+  int64_t nseqs;       // number of sequences (ncol * nbatch); sequence q -> (c = q % ncol, b = q / ncol)
+  int32_t ncol;
+  int32_t load_slot_fast, store_slot_fast;  // coalescing order of the global accesses
+  int32_t n;           // user length (prime p for Rader)
+  int32_t m;           // inner transform length (p-1 for Rader, M for Bluestein, n/2 for real-even)
+  int32_t C;           // sequences per workgroup
+  int32_t tpu;
+  int32_t lds_stride;  // complex slots per sequence in LDS
+  int32_t nst;
+  int32_t rad[kMaxStages];
+  FastDiv fd_q[kMaxStages];
+  int32_t twq_step[kMaxStages];
+  FastDiv fd_C, fd_ncol, fd_m;
+  int32_t lext, sext;    // elements per sequence read in the load phase / written in the store phase
+  FastDiv fd_lext, fd_sext;
+  const void *tw_small;  // w_m^j, j < m
+  const void *aux;       // Rader: int32 index table g^q mod p; real-even: w_n^k, k <= n/2
+  const void *bhat;      // FFT_m of the convolution kernel, pre-scaled by 1/m
+  const void *chirp;     // Bluestein chirp (n entries)
+  int32_t src_mode, dst_mode;
+  int32_t conj_in, conj_out;
+  double scale;
+};
+
+// Split-spectrum pre/post pass for even-length real transforms that do not fit one LDS transform.
+struct RealPassArgs {
+  const void *in;
+  void *out;
+  Geometry gi, go;
+  int64_t nseqs;
+  int32_t ncol;
+  int32_t n;            // real length (even)
+  const void *tw_half;  // w_n^k, k <= n/2
+  int32_t conj_flag;
+};
+
+}  // namespace flagfft

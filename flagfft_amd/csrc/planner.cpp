@@ -1,0 +1,331 @@
+// planner.cpp — decomposition choice (host only).
+#include "planner.hpp"
+
+#include <algorithm>
+#include <sstream>
+#include <stdexcept>
+
+namespace flagfft {
+
+bool is_prime(int64_t n) {
+  if (n < 2) return false;
+  if (n % 2 == 0) return n == 2;
+  for (int64_t d = 3; d * d <= n; d += 2)
+    if (n % d == 0) return false;
+  return true;
+}
+
+namespace {
+
+int64_t modpow(int64_t b, int64_t e, int64_t m) {
+  __int128 r = 1, x = b % m;
+  while (e > 0) {
+    if (e & 1) r = (r * x) % m;
+    x = (x * x) % m;
+    e >>= 1;
+  }
+  return static_cast<int64_t>(r);
+}
+
+std::vector<int64_t> prime_factors(int64_t n) {
+  std::vector<int64_t> f;
+  for (int64_t d = 2; d * d <= n; ++d)
+    while (n % d == 0) {
+      f.push_back(d);
+      n /= d;
+    }
+  if (n > 1) f.push_back(n);
+  return f;
+}
+
+std::string join(const std::vector<int> &v, const char *sep) {
+  std::ostringstream o;
+  for (size_t i = 0; i < v.size(); ++i) o << (i ? sep : "") << v[i];
+  return o.str();
+}
+
+std::string join64(const std::vector<int64_t> &v, const char *sep) {
+  std::ostringstream o;
+  for (size_t i = 0; i < v.size(); ++i) o << (i ? sep : "") << v[i];
+  return o.str();
+}
+
+}  // namespace
+
+// Smallest primitive root, as src/plan/rader_utils.cpp:55-77 chooses it.
+int64_t primitive_root(int64_t p) {
+  if (!is_prime(p)) return -1;
+  if (p == 2) return 1;
+  std::vector<int64_t> f = prime_factors(p - 1);
+  f.erase(std::unique(f.begin(), f.end()), f.end());
+  for (int64_t g = 2; g < p; ++g) {
+    bool ok = true;
+    for (int64_t q : f)
+      if (modpow(g, (p - 1) / q, p) == 1) {
+        ok = false;
+        break;
+      }
+    if (ok) return g;
+  }
+  return -1;
+}
+
+bool smooth13(int64_t n) {
+  if (n < 1) return false;
+  for (int64_t p : {2, 3, 5, 7, 11, 13})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// First-fit-decreasing packing of the prime factors into radices <= 16.
+bool radix_factor(int64_t n, std::vector<int> &out) {
+  out.clear();
+  if (n < 1) return false;
+  if (n == 1) return true;
+  std::vector<int64_t> f = prime_factors(n);
+  for (int64_t p : f)
+    if (p > 13) return false;
+  std::sort(f.begin(), f.end(), std::greater<int64_t>());
+  std::vector<int> bins;
+  for (int64_t p : f) {
+    bool placed = false;
+    for (int &b : bins)
+      if (b * p <= 16) {
+        b *= static_cast<int>(p);
+        placed = true;
+        break;
+      }
+    if (!placed) bins.push_back(static_cast<int>(p));
+  }
+  std::sort(bins.begin(), bins.end(), std::greater<int>());
+  out = bins;
+  return true;
+}
+
+int64_t leaf_max(const PlanOptions &o, Prec p) { return p == Prec::F32 ? o.leaf_max_f32 : o.leaf_max_f64; }
+int64_t pass_max(const PlanOptions &o, Prec p) { return p == Prec::F32 ? o.pass_max_f32 : o.pass_max_f64; }
+
+namespace {
+
+// Split n into k factors, each <= cap and radix-factorable; maximise the smallest factor.
+bool split_passes(int64_t n, int k, int64_t cap, std::vector<int64_t> &best) {
+  std::vector<int> tmp;
+  if (k == 1) {
+    if (n <= cap && radix_factor(n, tmp)) {
+      best = {n};
+      return true;
+    }
+    return false;
+  }
+  bool found = false;
+  int64_t best_min = 0;
+  for (int64_t d = 2; d <= cap && d <= n; ++d) {
+    if (n % d != 0 || !radix_factor(d, tmp)) continue;
+    std::vector<int64_t> rest;
+    if (!split_passes(n / d, k - 1, cap, rest)) continue;
+    int64_t mn = std::min(d, *std::min_element(rest.begin(), rest.end()));
+    if (!found || mn > best_min) {
+      found = true;
+      best_min = mn;
+      best.clear();
+      best.push_back(d);
+      best.insert(best.end(), rest.begin(), rest.end());
+    }
+  }
+  if (found) std::sort(best.begin(), best.end(), std::greater<int64_t>());
+  return found;
+}
+
+bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c) {
+  std::vector<int64_t> passes;
+  const int64_t cap = pass_max(o, p);
+  bool ok = false;
+  if (!o.forced_passes.empty()) {
+    int64_t prod = 1;
+    bool valid = true;
+    std::vector<int> tmp;
+    for (int64_t r : o.forced_passes) {
+      prod *= r;
+      valid = valid && r >= 2 && r <= leaf_max(o, p) && radix_factor(r, tmp);
+    }
+    if (valid && prod == n) {
+      passes = o.forced_passes;
+      ok = true;
+    }
+  }
+  for (int k = 2; !ok && k <= 4; ++k) ok = split_passes(n, k, cap, passes);
+  if (!ok) return false;
+  c.kind = CoreKind::MultiPass;
+  c.n = n;
+  c.passes = passes;
+  c.pass_radices.clear();
+  for (int64_t r : passes) {
+    std::vector<int> rad;
+    radix_factor(r, rad);
+    c.pass_radices.push_back(rad);
+  }
+  return true;
+}
+
+int rader_lds_bytes(int64_t p, Prec prec) {
+  const int64_t m = p - 1;
+  return static_cast<int>((lds_pad(static_cast<int>(p)) + 1 + lds_pad(static_cast<int>(m)) + 1 + 1) *
+                          elem_bytes(prec));
+}
+
+}  // namespace
+
+Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
+  if (n < 1) throw std::invalid_argument("transform length must be positive");
+  Core c;
+  c.n = n;
+  std::vector<int> rad;
+  const int64_t lmax = leaf_max(o, p);
+  if (n <= lmax && radix_factor(n, rad)) {
+    c.kind = CoreKind::Leaf;
+    if (!o.forced_radices.empty()) {
+      int64_t prod = 1;
+      for (int r : o.forced_radices) prod *= r;
+      bool valid = prod == n;
+      for (int r : o.forced_radices) valid = valid && r >= 2 && r <= 16;
+      if (valid) rad = o.forced_radices;
+    }
+    c.radices = rad;
+    return c;
+  }
+  if (is_prime(n) && n > 16) {
+    const int64_t m = n - 1;
+    if (m <= lmax && radix_factor(m, rad) && rader_lds_bytes(n, p) <= o.lds_cap_bytes) {
+      c.kind = CoreKind::Rader;
+      c.m = m;
+      c.root = primitive_root(n);
+      c.radices = rad;
+      return c;
+    }
+  }
+  if (smooth13(n) && multipass_core(n, p, o, c)) return c;
+  // Bluestein: smallest 13-smooth convolution length M >= 2n-1 that plans.
+  for (int64_t M = 2 * n - 1;; ++M) {
+    if (!smooth13(M)) continue;
+    if (M <= lmax && radix_factor(M, rad)) {
+      c.kind = CoreKind::Bluestein;
+      c.m = M;
+      c.radices = rad;
+      c.conv.reset();
+      return c;
+    }
+    Core sub;
+    if (multipass_core(M, p, o, sub)) {
+      c.kind = CoreKind::Bluestein;
+      c.m = M;
+      c.radices.clear();
+      c.conv = std::make_shared<Core>(sub);
+      return c;
+    }
+    if (M > 64 * n + 1024) throw std::runtime_error("no Bluestein convolution length found");
+  }
+}
+
+std::string Core::describe(int indent) const {
+  std::ostringstream o;
+  std::string pad(static_cast<size_t>(indent), ' ');
+  switch (kind) {
+    case CoreKind::Leaf:
+      o << pad << "Leaf(n=" << n << ", radices=[" << join(radices, ",") << "], passes=1)";
+      break;
+    case CoreKind::MultiPass:
+      o << pad << "MultiPassStockham(n=" << n << ", super_radices=[" << join64(passes, ",") << "], passes="
+        << passes.size() << ")";
+      for (size_t i = 0; i < passes.size(); ++i)
+        o << "\n" << pad << "  pass" << i << ": R=" << passes[i] << " radices=[" << join(pass_radices[i], ",") << "]";
+      break;
+    case CoreKind::Rader:
+      o << pad << "Rader(p=" << n << ", root=" << root << ", conv_length=" << m << ", fused=1)\n"
+        << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
+      break;
+    case CoreKind::Bluestein:
+      o << pad << "Bluestein(n=" << n << ", conv_length=" << m << ", fused=" << (conv ? 0 : 1) << ")\n";
+      if (conv) o << conv->describe(indent + 2);
+      else o << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
+      break;
+  }
+  return o.str();
+}
+
+std::string Plan1D::describe(int indent) const {
+  std::ostringstream o;
+  std::string pad(static_cast<size_t>(indent), ' ');
+  const char *form = "";
+  switch (real_form) {
+    case RealForm::None: form = "Complex"; break;
+    case RealForm::EvenFused: form = "RealEvenFused"; break;
+    case RealForm::EvenMulti: form = "RealEvenSplitPass"; break;
+    case RealForm::Full: form = "RealPromoted"; break;
+  }
+  o << pad << form << "(n=" << n << (real_fwd ? ", R2C" : "") << (real_inv ? ", C2R" : "") << ")\n"
+    << core.describe(indent + 2);
+  return o.str();
+}
+
+namespace {
+
+Plan1D plan_1d(int64_t n, bool real_fwd, bool real_inv, Prec p, const PlanOptions &o) {
+  Plan1D pl;
+  pl.n = n;
+  pl.real_fwd = real_fwd;
+  pl.real_inv = real_inv;
+  if (!real_fwd && !real_inv) {
+    pl.real_form = RealForm::None;
+    pl.core = plan_core(n, p, o);
+    return pl;
+  }
+  if (n % 2 == 0 && n >= 2) {
+    Core half = plan_core(n / 2, p, o);
+    if (half.kind == CoreKind::Leaf) {
+      pl.real_form = RealForm::EvenFused;
+      pl.core = half;
+      return pl;
+    }
+    if (half.kind == CoreKind::MultiPass) {
+      pl.real_form = RealForm::EvenMulti;
+      pl.core = half;
+      return pl;
+    }
+  }
+  pl.real_form = RealForm::Full;
+  pl.core = plan_core(n, p, o);
+  return pl;
+}
+
+}  // namespace
+
+PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
+  PlanTree t;
+  t.desc = d;
+  if (d.rank == 1) {
+    t.rows = plan_1d(d.n0, d.real_fwd, d.real_inv, d.prec, o);
+  } else {
+    t.rows = plan_1d(d.n1, d.real_fwd, d.real_inv, d.prec, o);
+    t.cols = plan_1d(d.n0, false, false, d.prec, o);
+  }
+  return t;
+}
+
+std::string PlanTree::describe() const {
+  std::ostringstream o;
+  o << "=== FlagFFT Plan (gfx950) ===\n";
+  o << "rank=" << desc.rank << " n=[" << desc.n0;
+  if (desc.rank == 2) o << "," << desc.n1;
+  o << "] batch=" << desc.batch << " type=" << desc.type << " precision=" << (desc.prec == Prec::F32 ? "fp32" : "fp64")
+    << "\n\n-- Plan tree --\n";
+  if (desc.rank == 1) {
+    o << rows.describe(0) << "\n";
+  } else {
+    o << "TwoDim(n0=" << desc.n0 << ", n1=" << desc.n1 << ", order=rows-then-columns, column_pass=strided-tile)\n";
+    o << "  rows (axis 1):\n" << rows.describe(4) << "\n";
+    o << "  columns (axis 0):\n" << cols.describe(4) << "\n";
+  }
+  return o.str();
+}
+
+}  // namespace flagfft

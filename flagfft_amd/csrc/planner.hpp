@@ -1,0 +1,92 @@
+// planner.hpp — decomposition choice for the MI355X FFT path (host only, no HIP).
+//
+// Mirrors the role of the reference's src/plan (PlanBuilder::build,
+// build_auto_candidates, select_leaf_factors: src/plan/builder_context.cpp:42-67,
+// src/plan/auto_candidates.cpp:56-106, src/plan/factorization.cpp:70-193) for a
+// different machine: the decision is between a single LDS-resident pass, a
+// multi-pass Stockham super-stage chain, fused Rader and Bluestein, with
+// real transforms split into an n/2-point complex core where possible.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace flagfft {
+
+enum class Prec { F32, F64 };
+
+inline int elem_bytes(Prec p) { return p == Prec::F32 ? 8 : 16; }
+
+// Tunables; defaults are the untuned heuristics, the tuned-plan DB may override.
+struct PlanOptions {
+  int64_t leaf_max_f32 = 8192;    // longest single-pass LDS transform
+  int64_t leaf_max_f64 = 4096;
+  int64_t pass_max_f32 = 4096;    // longest super-radix of a multi-pass chain
+  int64_t pass_max_f64 = 2048;
+  int lds_pref_bytes = 64 * 1024;   // preferred LDS per workgroup
+  int lds_cap_bytes = 160 * 1024;   // hard cap (gfx950 LDS per CU)
+  int target_threads = 256;         // workgroup size aimed for by single-pass rows
+  int chunk_bytes = 128;            // contiguous bytes per row segment for strided passes
+  int ept = 0;                      // 0 = automatic (16, or 32 when needed)
+  int xcd_remap = 1;
+  std::vector<int64_t> forced_passes;  // explicit super-radix split (tuned plans)
+  std::vector<int> forced_radices;     // explicit single-pass radix order (tuned plans)
+  int forced_units = 0;                // explicit units per workgroup (tuned plans)
+};
+
+enum class CoreKind { Leaf, MultiPass, Rader, Bluestein };
+
+// A length-n complex transform of a batch of sequences.
+struct Core {
+  CoreKind kind = CoreKind::Leaf;
+  int64_t n = 0;
+  std::vector<int> radices;                  // Leaf; Rader/Bluestein fused inner transform
+  std::vector<int64_t> passes;               // MultiPass super-radices
+  std::vector<std::vector<int>> pass_radices;
+  int64_t m = 0;                             // Rader: p-1; Bluestein: M
+  int64_t root = 0;                          // Rader primitive root
+  std::shared_ptr<Core> conv;                // Bluestein: multi-pass core of M (null when fused)
+  std::string describe(int indent = 0) const;
+};
+
+enum class RealForm { None, EvenFused, EvenMulti, Full };
+
+struct Plan1D {
+  int64_t n = 0;
+  bool real_fwd = false, real_inv = false;
+  RealForm real_form = RealForm::None;
+  Core core;  // length n, or n/2 for EvenFused / EvenMulti
+  std::string describe(int indent = 0) const;
+};
+
+struct PlanDesc {
+  int rank = 1;
+  int64_t n0 = 0, n1 = 0;     // rank 1: n0 = n
+  int type = 0;                // flagfftType value
+  int64_t batch = 1;
+  Prec prec = Prec::F32;
+  bool real_fwd = false, real_inv = false;
+};
+
+struct PlanTree {
+  PlanDesc desc;
+  Plan1D rows;   // rank 1: the transform; rank 2: along n1 (contiguous)
+  Plan1D cols;   // rank 2: C2C along n0 (strided)
+  std::string describe() const;
+};
+
+// Helpers exposed for tests and the executor.
+bool is_prime(int64_t n);
+int64_t primitive_root(int64_t p);                  // smallest, -1 if p not prime
+bool radix_factor(int64_t n, std::vector<int> &out);  // small radices 2..16 (product n)
+bool smooth13(int64_t n);
+int64_t leaf_max(const PlanOptions &o, Prec p);
+int64_t pass_max(const PlanOptions &o, Prec p);
+inline int lds_pad(int e) { return e + (e >> 5); }   // must match the kernels' lpad()
+
+Core plan_core(int64_t n, Prec p, const PlanOptions &o);
+PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o);
+
+}  // namespace flagfft

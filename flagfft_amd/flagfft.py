@@ -1,0 +1,300 @@
+"""ctypes mirror of the C API in include/flagfft.h (and include/flagfft_ext.h).
+
+Same names, argument meaning and result codes as the reference's C API
+(include/flagfft.h:1-86 of the reference); device buffers are passed as raw
+device pointers (ints), e.g. ``tensor.data_ptr()`` of a torch tensor on the GPU.
+The library is the in-tree libflagfft.so; importing this module raises if it
+has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libflagfft.so")
+
+
+class Result(enum.IntEnum):
+    SUCCESS = 0
+    INVALID_PLAN = 1
+    ALLOC_FAILED = 2
+    INVALID_TYPE = 3
+    INVALID_VALUE = 4
+    INTERNAL_ERROR = 5
+    EXEC_FAILED = 6
+    SETUP_FAILED = 7
+    INVALID_SIZE = 8
+    UNALIGNED_DATA = 9
+    INCOMPLETE_PARAMETER_LIST = 10
+    INVALID_DEVICE = 11
+    PARSE_ERROR = 12
+    NO_WORKSPACE = 13
+    NOT_SUPPORTED = 14
+
+
+class Type(enum.IntEnum):
+    R2C = 0x2A
+    C2R = 0x2C
+    C2C = 0x29
+    D2Z = 0x6A
+    Z2D = 0x6C
+    Z2Z = 0x69
+
+
+FORWARD = -1
+INVERSE = 1
+
+SYMBOLS = [
+    "flagfftPlan1d", "flagfftPlan2d", "flagfftPlan3d", "flagfftPlanMany",
+    "flagfftExecC2C", "flagfftExecZ2Z", "flagfftExecR2C", "flagfftExecD2Z", "flagfftExecC2R", "flagfftExecZ2D",
+    "flagfftSetStream", "flagfftDestroy", "flagfftGetPlanDescription",
+]
+EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning"]
+
+
+class FlagFFTError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        try:
+            name = Result(code).name
+        except ValueError:
+            name = str(code)
+        super().__init__(f"{fn} failed with FLAGFFT_{name}")
+        self.code = code
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("leaf_max", ctypes.c_int), ("pass_max", ctypes.c_int), ("units", ctypes.c_int), ("ept", ctypes.c_int),
+        ("chunk_bytes", ctypes.c_int), ("npasses", ctypes.c_int), ("passes", ctypes.c_int * 4),
+        ("nradices", ctypes.c_int), ("radices", ctypes.c_int * 16),
+    ]
+
+
+_lib = None
+_H = ctypes.c_void_p
+_HP = ctypes.POINTER(ctypes.c_void_p)
+_IP = ctypes.POINTER(ctypes.c_int)
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load libflagfft.so (raises FileNotFoundError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        sigs = {
+            "flagfftPlan1d": [_HP, _int, _int, _int],
+            "flagfftPlan2d": [_HP, _int, _int, _int],
+            "flagfftPlan3d": [_HP, _int, _int, _int, _int],
+            "flagfftPlanMany": [_HP, _int, _IP, _IP, _int, _int, _IP, _int, _int, _int, _int],
+            "flagfftExecC2C": [_H, _vp, _vp, _int],
+            "flagfftExecZ2Z": [_H, _vp, _vp, _int],
+            "flagfftExecR2C": [_H, _vp, _vp],
+            "flagfftExecD2Z": [_H, _vp, _vp],
+            "flagfftExecC2R": [_H, _vp, _vp],
+            "flagfftExecZ2D": [_H, _vp, _vp],
+            "flagfftSetStream": [_H, _vp],
+            "flagfftDestroy": [_H],
+            "flagfftxPlanDescribe": [_int, _IP, _int, _int, ctypes.c_char_p, ctypes.c_size_t],
+            "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
+            "flagfftxSetTuning": [ctypes.POINTER(Tuning)],
+        }
+        for name, argtypes in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = _int
+        L.flagfftGetPlanDescription.argtypes = [_H]
+        L.flagfftGetPlanDescription.restype = ctypes.c_char_p
+        L.flagfftxVersion.argtypes = []
+        L.flagfftxVersion.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _arr(values):
+    if values is None:
+        return None
+    return (ctypes.c_int * len(values))(*values)
+
+
+# ---- raw calls returning result codes (mirror the C signatures) ------------
+def plan1d(nx: int, type_: int, batch: int):
+    h = ctypes.c_void_p()
+    r = lib().flagfftPlan1d(ctypes.byref(h), nx, int(type_), batch)
+    return r, h
+
+
+def plan2d(nx: int, ny: int, type_: int):
+    h = ctypes.c_void_p()
+    r = lib().flagfftPlan2d(ctypes.byref(h), nx, ny, int(type_))
+    return r, h
+
+
+def plan3d(nx: int, ny: int, nz: int, type_: int):
+    h = ctypes.c_void_p()
+    r = lib().flagfftPlan3d(ctypes.byref(h), nx, ny, nz, int(type_))
+    return r, h
+
+
+def plan_many(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch):
+    h = ctypes.c_void_p()
+    r = lib().flagfftPlanMany(ctypes.byref(h), rank, _arr(n), _arr(inembed), istride, idist, _arr(onembed), ostride,
+                              odist, int(type_), batch)
+    return r, h
+
+
+def exec_c2c(h, idata: int, odata: int, direction: int) -> int:
+    return lib().flagfftExecC2C(h, idata, odata, direction)
+
+
+def exec_z2z(h, idata: int, odata: int, direction: int) -> int:
+    return lib().flagfftExecZ2Z(h, idata, odata, direction)
+
+
+def exec_r2c(h, idata: int, odata: int) -> int:
+    return lib().flagfftExecR2C(h, idata, odata)
+
+
+def exec_d2z(h, idata: int, odata: int) -> int:
+    return lib().flagfftExecD2Z(h, idata, odata)
+
+
+def exec_c2r(h, idata: int, odata: int) -> int:
+    return lib().flagfftExecC2R(h, idata, odata)
+
+
+def exec_z2d(h, idata: int, odata: int) -> int:
+    return lib().flagfftExecZ2D(h, idata, odata)
+
+
+def set_stream(h, stream: int) -> int:
+    return lib().flagfftSetStream(h, stream)
+
+
+def destroy(h) -> int:
+    return lib().flagfftDestroy(h)
+
+
+def description(h) -> str | None:
+    s = lib().flagfftGetPlanDescription(h)
+    return s.decode() if s is not None else None
+
+
+def describe_host(rank: int, n, type_: int, batch: int = 1):
+    """Planner-only description (no device is touched): (code, text)."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    r = lib().flagfftxPlanDescribe(rank, _arr(n), int(type_), batch, buf, len(buf))
+    return r, buf.value.decode()
+
+
+def plan_stats(h, direction: int = FORWARD):
+    launches = ctypes.c_int(0)
+    nbytes = ctypes.c_longlong(0)
+    r = lib().flagfftxPlanStats(h, direction, ctypes.byref(launches), ctypes.byref(nbytes))
+    return r, launches.value, nbytes.value
+
+
+def set_tuning(**kw) -> int:
+    """Plan-time tunables for plans created afterwards on this thread; no args restores defaults."""
+    if not kw:
+        return lib().flagfftxSetTuning(None)
+    t = Tuning()
+    for k, v in kw.items():
+        if k == "passes":
+            t.npasses = len(v)
+            for i, x in enumerate(v):
+                t.passes[i] = x
+        elif k == "radices":
+            t.nradices = len(v)
+            for i, x in enumerate(v):
+                t.radices[i] = x
+        else:
+            setattr(t, k, v)
+    return lib().flagfftxSetTuning(ctypes.byref(t))
+
+
+def version() -> str:
+    return lib().flagfftxVersion().decode()
+
+
+def check(fn: str, code: int):
+    if code != Result.SUCCESS:
+        raise FlagFFTError(fn, code)
+
+
+# ---- convenience object --------------------------------------------------
+class Plan:
+    """Owning wrapper of a flagfftHandle (raises FlagFFTError on failure)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def one_d(cls, n: int, type_: int, batch: int = 1):
+        r, h = plan1d(n, type_, batch)
+        check("flagfftPlan1d", r)
+        return cls(h)
+
+    @classmethod
+    def two_d(cls, nx: int, ny: int, type_: int):
+        r, h = plan2d(nx, ny, type_)
+        check("flagfftPlan2d", r)
+        return cls(h)
+
+    @classmethod
+    def many(cls, rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch):
+        r, h = plan_many(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch)
+        check("flagfftPlanMany", r)
+        return cls(h)
+
+    def set_stream(self, stream: int):
+        check("flagfftSetStream", set_stream(self.handle, stream))
+
+    def c2c(self, i: int, o: int, direction: int = FORWARD):
+        check("flagfftExecC2C", exec_c2c(self.handle, i, o, direction))
+
+    def z2z(self, i: int, o: int, direction: int = FORWARD):
+        check("flagfftExecZ2Z", exec_z2z(self.handle, i, o, direction))
+
+    def r2c(self, i: int, o: int):
+        check("flagfftExecR2C", exec_r2c(self.handle, i, o))
+
+    def d2z(self, i: int, o: int):
+        check("flagfftExecD2Z", exec_d2z(self.handle, i, o))
+
+    def c2r(self, i: int, o: int):
+        check("flagfftExecC2R", exec_c2r(self.handle, i, o))
+
+    def z2d(self, i: int, o: int):
+        check("flagfftExecZ2D", exec_z2d(self.handle, i, o))
+
+    def description(self) -> str:
+        return description(self.handle)
+
+    def stats(self, direction: int = FORWARD):
+        r, launches, nbytes = plan_stats(self.handle, direction)
+        check("flagfftxPlanStats", r)
+        return launches, nbytes
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            check("flagfftDestroy", destroy(self.handle))
+        self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value:
+                destroy(self.handle)
+        except Exception:
+            pass

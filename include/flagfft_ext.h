@@ -1,0 +1,50 @@
+/*
+ * flagfft_ext.h — MI355X-build extensions beside the drop-in API of flagfft.h.
+ *
+ * None of these has a counterpart in the reference's include/flagfft.h; they
+ * expose what this build adds (host-side plan inspection for tests and tools,
+ * per-plan launch statistics for byte accounting, and tuning overrides).
+ */
+#pragma once
+
+#include <stddef.h>
+
+#include "flagfft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version string ("flagfft-gfx950 <major>.<minor>"). */
+const char *flagfftxVersion(void);
+
+/* Run only the planner (no device is touched) for the same arguments that
+ * flagfftPlanMany would validate with default (contiguous) layouts, and write
+ * the plan tree text into buf (truncated to len-1 bytes, NUL-terminated).
+ * Returns the same validation codes as flagfftPlanMany (reference:
+ * src/exec/c_api_plan.cpp:274-336, src/exec/c_api_internal.cpp:60-120). */
+flagfftResult flagfftxPlanDescribe(int rank, const int *n, flagfftType type, int batch, char *buf, size_t len);
+
+/* Number of kernel launches one Exec of `direction` enqueues and the sum of
+ * their staged HBM bytes (read + write of every pass, workspaces included). */
+flagfftResult flagfftxPlanStats(flagfftHandle plan, int direction, int *launches, long long *pass_bytes);
+
+/* Plan-time tunables for plans created afterwards by this thread (0 keeps
+ * the default).  Used by the autotuner to sweep decompositions. */
+typedef struct flagfftxTuning_t {
+  int leaf_max;       /* longest single-pass LDS transform */
+  int pass_max;       /* longest super-radix of a multi-pass chain */
+  int units;          /* transforms (or butterflies) per workgroup */
+  int ept;            /* elements per thread: 16 or 32 (fp32), 8 or 16 (fp64) */
+  int chunk_bytes;    /* contiguous bytes per row segment in strided passes */
+  int npasses;        /* explicit super-radix split, `npasses` entries of `passes` */
+  int passes[4];
+  int nradices;       /* explicit single-pass radix order */
+  int radices[16];
+} flagfftxTuning;
+
+flagfftResult flagfftxSetTuning(const flagfftxTuning *tuning); /* NULL restores defaults */
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,172 @@
+"""C-ABI boundary and host logic, no GPU needed.
+
+* libflagfft.so loads and exports every function declared in include/*.h;
+* descriptor validation returns the reference's codes before touching a device
+  (ctest/test_plan.cpp; src/exec/c_api_plan.cpp:274-336; c_api_internal.cpp:60-120);
+* the planner (flagfftxPlanDescribe, host only) produces decompositions whose
+  integer invariants hold (products, radix set, LDS budget, Rader/Bluestein choice).
+"""
+import math
+import os
+import re
+
+import pytest
+
+from flagfft_amd import flagfft as ff
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+R = ff.Result
+T = ff.Type
+ALL_TYPES = [T.C2C, T.Z2Z, T.R2C, T.D2Z, T.C2R, T.Z2D]
+
+
+def _declared_functions():
+    names = []
+    for h in ("flagfft.h", "flagfft_ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(flagfft\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ff.lib()
+    declared = _declared_functions()
+    assert set(ff.SYMBOLS) <= set(declared) and set(ff.EXT_SYMBOLS) <= set(declared)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert ff.version().startswith("flagfft-gfx950")
+
+
+def test_enum_values_match_cufft_layout():
+    """include/flagfft.h:28-35 of the reference uses cuFFT's values."""
+    assert (T.R2C, T.C2R, T.C2C, T.D2Z, T.Z2D, T.Z2Z) == (0x2A, 0x2C, 0x29, 0x6A, 0x6C, 0x69)
+    assert ff.FORWARD == -1 and ff.INVERSE == 1
+    assert [r.value for r in R] == list(range(15))
+
+
+def test_plan_invalid_parameters():
+    """ctest/test_plan.cpp:26-36, 126-131, 155-169."""
+    r, h = ff.plan1d(0, T.C2C, 1)
+    assert r == R.INVALID_SIZE and not h.value
+    assert ff.lib().flagfftPlan1d(None, 256, int(T.C2C), 1) == R.INVALID_VALUE
+    r, h = ff.plan1d(256, T.C2C, 0)
+    assert r == R.INVALID_VALUE and not h.value
+    r, h = ff.plan2d(0, 32, T.C2C)
+    assert r == R.INVALID_SIZE and not h.value
+    assert ff.lib().flagfftPlan2d(None, 64, 32, int(T.C2C)) == R.INVALID_VALUE
+    for t in ALL_TYPES:
+        r, h = ff.plan3d(32, 16, 8, t)
+        assert r == R.NOT_SUPPORTED and not h.value
+    r, h = ff.plan3d(0, 16, 8, T.C2C)
+    assert r == R.INVALID_SIZE
+    assert ff.lib().flagfftPlan3d(None, 32, 16, 8, int(T.C2C)) == R.INVALID_VALUE
+    r, h = ff.plan1d(256, 0x33, 1)
+    assert r == R.INVALID_TYPE and not h.value
+    r, h = ff.plan_many(0, [16], None, 1, 16, None, 1, 16, T.C2C, 1)
+    assert r == R.INVALID_VALUE
+
+
+def test_layouts_outside_the_contract_are_not_supported():
+    """2D istride != 1 and padded 2D dist (ctest/test_plan.cpp:105-124); 1D strides."""
+    n = [64, 32]
+    logical = 64 * 32
+    assert ff.plan_many(2, n, None, 2, logical, None, 1, logical, T.C2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many(2, n, None, 1, logical + 16, None, 1, logical, T.C2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many(2, n, None, 1, logical, None, 1, logical + 16, T.C2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many(1, [256], None, 2, 256, None, 1, 256, T.C2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many(1, [256], [256], 1, 300, [256], 1, 256, T.C2C, 2)[0] == R.NOT_SUPPORTED
+
+
+def test_valid_descriptors_reach_the_device_check():
+    """Without a GPU a valid descriptor passes validation and stops at INVALID_DEVICE."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    for t in ALL_TYPES:
+        assert ff.plan1d(256, t, 1)[0] == R.INVALID_DEVICE
+    # padded in-place real layouts (src/exec/c_api_internal.cpp:74-89)
+    assert ff.plan_many(1, [256], None, 1, 258, None, 1, 129, T.R2C, 4)[0] == R.INVALID_DEVICE
+    assert ff.plan_many(1, [256], None, 1, 129, None, 1, 258, T.C2R, 4)[0] == R.INVALID_DEVICE
+    assert ff.plan2d(64, 32, T.R2C)[0] == R.INVALID_DEVICE
+
+
+def test_null_handles():
+    L = ff.lib()
+    assert L.flagfftDestroy(None) == R.INVALID_PLAN
+    assert L.flagfftSetStream(None, None) == R.INVALID_PLAN
+    assert L.flagfftGetPlanDescription(None) is None
+    assert L.flagfftExecC2C(None, 8, 8, ff.FORWARD) == R.INVALID_PLAN
+    assert L.flagfftExecR2C(None, 8, 8) == R.INVALID_PLAN
+    assert L.flagfftxPlanStats(None, -1, None, None) == R.INVALID_PLAN
+
+
+# ---- planner invariants -------------------------------------------------------
+def _describe(n, t=T.C2C, batch=1, rank=1):
+    r, s = ff.describe_host(rank, n if isinstance(n, list) else [n], t, batch)
+    assert r == R.SUCCESS, (n, r)
+    return s
+
+
+def _core_checks(n, text):
+    leaf = re.search(r"Leaf\(n=(\d+), radices=\[([\d,]*)\]", text)
+    multi = re.search(r"MultiPassStockham\(n=(\d+), super_radices=\[([\d,]+)\]", text)
+    if "Rader(" in text:
+        m = re.search(r"Rader\(p=(\d+), root=(\d+), conv_length=(\d+)", text)
+        p, g, conv = map(int, m.groups())
+        assert conv == p - 1 and all(p % d for d in range(2, int(p ** 0.5) + 1))
+        assert len({pow(g, e, p) for e in range(1, p)}) == p - 1
+    elif "Bluestein(" in text:
+        m = re.search(r"Bluestein\(n=(\d+), conv_length=(\d+)", text)
+        nn, conv = map(int, m.groups())
+        assert conv >= 2 * nn - 1
+        c = conv
+        for q in (2, 3, 5, 7, 11, 13):
+            while c % q == 0:
+                c //= q
+        assert c == 1
+    if multi and "Bluestein" not in text:
+        nn = int(multi.group(1))
+        parts = [int(v) for v in multi.group(2).split(",")]
+        assert math.prod(parts) == nn and all(2 <= v <= 4096 for v in parts)
+    for m in re.finditer(r"(?<!super_)radices=\[([\d,]*)\]", text):
+        rad = [int(v) for v in m.group(1).split(",") if v]
+        assert all(2 <= v <= 16 for v in rad)
+    if leaf and "Rader" not in text and "Bluestein" not in text and not multi:
+        rad = [int(v) for v in leaf.group(2).split(",") if v]
+        assert math.prod(rad) == int(leaf.group(1))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 23, 64, 67, 81, 97, 243, 256, 361, 512, 997, 1009, 1024, 2048,
+                               4096, 4099, 6000, 8192, 10007, 10935, 16384, 15015, 65536, 1 << 20, 1 << 24,
+                               2 * 4999, 30011])
+def test_planner_invariants(n):
+    for t in (T.C2C, T.Z2Z):
+        _core_checks(n, _describe(n, t))
+
+
+def test_planner_choices():
+    assert "Leaf(n=4096, radices=[16,16,16]" in _describe(4096)
+    assert "Rader(p=67" in _describe(67)                      # ctest/test_plan.cpp:47-54
+    assert "Rader(p=67" in _describe(67, T.R2C)                # :56-61
+    assert "Rader(p=67" in _describe([67, 32], T.C2C, rank=2)  # :142-149
+    assert "Bluestein(n=997" in _describe(997)                 # 996 = 4*3*83: not smooth
+    assert "Rader(p=1009" in _describe(1009)                   # 1008 = 16*9*7
+    assert "RealEvenFused(n=6000" in _describe(6000, T.R2C)
+    assert "RealEvenSplitPass(n=32768" in _describe(32768, T.C2R)
+    assert "RealPromoted(n=243" in _describe(243, T.D2Z)
+    assert "super_radices=[4096,4096]" in _describe(1 << 24)
+    assert "TwoDim(n0=4096, n1=4096" in _describe([4096, 4096], T.R2C, rank=2)
+
+
+def test_tuning_overrides_reach_the_planner():
+    try:
+        assert ff.set_tuning(radices=[4, 16, 16, 4]) == R.SUCCESS
+        assert "radices=[4,16,16,4]" in _describe(4096)
+        assert ff.set_tuning(leaf_max=1024, passes=[64, 64]) == R.SUCCESS
+        assert "super_radices=[64,64]" in _describe(4096)
+        assert ff.set_tuning(ept=12) == R.INVALID_VALUE
+    finally:
+        ff.set_tuning()
+    assert "Leaf(n=4096, radices=[16,16,16]" in _describe(4096)

@@ -1,0 +1,103 @@
+"""C2C / Z2Z parity against the oracle through the C ABI (GPU).
+
+Mirrors the reference's 1D accuracy suite (ctest/test_exec_{c2c,z2z}_{fwd,inv}_{ct,bs}_{s,b}.cpp):
+its sizes, batches, input scales and roundtrip checks, with the FP64 oracle in place of cuFFT.
+"""
+import numpy as np
+import pytest
+
+from flagfft_amd import flagfft as ff
+from oracle import oracle as ora
+
+from .helpers import check_accuracy, complex_input, run_c2c
+
+pytestmark = pytest.mark.gpu
+
+CT_SIZES = [1, 2, 3, 5, 7, 11, 13, 16, 64, 81, 243, 256, 361, 512, 1024, 2048, 4096, 8192, 16384]
+BS_SIZES = [17, 23, 67, 97, 997, 1009, 4099]
+EXTRA = [6000, 10935, 10007, 3 * 5 * 7 * 11 * 13, 2 * 4999, 20000, 65536, 1 << 20]
+
+
+def _case(dev, n, batch, dtype, direction, scale=1.0, kind="uniform", inplace=False):
+    t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
+    seed = ora.accuracy_seed(int(t), n, batch, 0)
+    x = complex_input(seed, (batch, n), dtype, kind, scale)
+    y = run_c2c(dev, x, direction, n, batch, inplace=inplace)
+    ref = ora.fft(x.astype(np.complex128), sign=direction)
+    return check_accuracy(y, ref, t, n, row_len=n, what=f"dir={direction} batch={batch} scale={scale}")
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", CT_SIZES)
+def test_cooley_tukey_sizes(gpu, n, dtype):
+    for batch in (1, 4):
+        for direction in (ff.FORWARD, ff.INVERSE):
+            _case(gpu, n, batch, dtype, direction)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", BS_SIZES)
+def test_prime_sizes(gpu, n, dtype):
+    for batch in (1, 4):
+        for direction in (ff.FORWARD, ff.INVERSE):
+            _case(gpu, n, batch, dtype, direction)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", EXTRA)
+def test_multipass_and_bluestein_sizes(gpu, n, dtype):
+    batch = 2 if n < (1 << 20) else 1
+    for direction in (ff.FORWARD, ff.INVERSE):
+        _case(gpu, n, batch, dtype, direction)
+
+
+@pytest.mark.parametrize("scale", [2.0 ** -20, 2.0 ** 20])
+@pytest.mark.parametrize("n", [64, 997, 4096, 16384])
+def test_input_scales(gpu, n, scale):
+    """ctest/flagfft_test.h:457 scales inputs by 2^-20, 1, 2^20."""
+    _case(gpu, n, 4, np.complex64, ff.FORWARD, scale=scale)
+
+
+@pytest.mark.parametrize("n", [16, 4096, 16384, 67, 997, 10007, 10935])
+def test_roundtrip_is_n_times_x(gpu, n):
+    """inverse(forward(x)) = N x (ctest/test_exec_c2c_inv_bs_s.cpp:73-91)."""
+    for dtype, t in ((np.complex64, ff.Type.C2C), (np.complex128, ff.Type.Z2Z)):
+        x = complex_input(ora.accuracy_seed(int(t), n, 4, 1), (4, n), dtype)
+        y = run_c2c(gpu, x, ff.FORWARD, n, 4)
+        z = run_c2c(gpu, y, ff.INVERSE, n, 4)
+        check_accuracy(z, x.astype(np.complex128) * n, t, n, row_len=n, factor=2.0, what="roundtrip")
+
+
+@pytest.mark.parametrize("n", [256, 4096, 16384, 67, 997, 10007])
+def test_in_place(gpu, n):
+    for dtype in (np.complex64, np.complex128):
+        _case(gpu, n, 3, dtype, ff.FORWARD, inplace=True)
+        _case(gpu, n, 3, dtype, ff.INVERSE, inplace=True)
+
+
+def test_config_4096x256_forward_inverse(gpu):
+    """C2C FP32 N=4096 batch 256, N(0,1) inputs: forward vs FP64 oracle and inverse/N roundtrip <= 2.5e-5."""
+    n, batch = 4096, 256
+    x = complex_input(0x4096, (batch, n), np.complex64, kind="normal")
+    y = run_c2c(gpu, x, ff.FORWARD, n, batch)
+    z = run_c2c(gpu, y, ff.INVERSE, n, batch)
+    e_fwd, _ = ora.rel_l2(y, ora.fft(x.astype(np.complex128)))
+    e_rt, _ = ora.rel_l2(z / n, x)
+    assert e_fwd <= 2.5e-5 and e_rt <= 2.5e-5, (e_fwd, e_rt)
+
+
+@pytest.mark.parametrize("n", [10007, 10935])
+def test_config_arbitrary_lengths(gpu, n):
+    """C2C FP32 and Z2Z FP64 at N=10007 (prime) and 10935 = 3^7*5, batch 64, N(0,1)."""
+    for dtype, t in ((np.complex64, ff.Type.C2C), (np.complex128, ff.Type.Z2Z)):
+        x = complex_input(0xC0FFEE + n, (64, n), dtype, kind="normal")
+        y = run_c2c(gpu, x, ff.FORWARD, n, 64)
+        check_accuracy(y, ora.fft(x.astype(np.complex128)), t, n, row_len=n, what="config")
+
+
+def test_large_four_step(gpu):
+    """C2C FP32 N=2^20 batch 16 and N=2^24 batch 1 (multi-pass Stockham)."""
+    for n, batch in ((1 << 20, 16), (1 << 24, 1)):
+        x = complex_input(0xB16 + n, (batch, n), np.complex64, kind="normal")
+        y = run_c2c(gpu, x, ff.FORWARD, n, batch)
+        check_accuracy(y, ora.fft(x.astype(np.complex128)), ff.Type.C2C, n, row_len=n, what="large")
