@@ -356,6 +356,52 @@ extern "C" flagfftResult flagfftxPlanStats(flagfftHandle handle, int direction, 
   return FLAGFFT_SUCCESS;
 }
 
+namespace flagfft {
+namespace {
+Executable *exec_for(PlanImpl *p, int direction) {
+  if (p->desc.real_fwd) return p->fwd.get();
+  if (p->desc.real_inv) return p->inv.get();
+  return direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
+}
+}  // namespace
+}  // namespace flagfft
+
+extern "C" flagfftResult flagfftxSetProfiling(flagfftHandle handle, int enable) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (p->fwd) p->fwd->set_profiling(enable != 0);
+  if (p->inv) p->inv->set_profiling(enable != 0);
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" flagfftResult flagfftxGetProfile(flagfftHandle handle, int direction, int capacity, double *ms_total,
+                                           long long *calls, long long *bytes, int *nsteps) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  if (nsteps == nullptr || capacity < 0) return FLAGFFT_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(p->mu);
+  flagfft::Executable *ex = flagfft::exec_for(p, direction);
+  if (ex == nullptr) return FLAGFFT_INVALID_VALUE;
+  if (ex->harvest_profile() != hipSuccess) return FLAGFFT_EXEC_FAILED;
+  *nsteps = static_cast<int>(ex->steps.size());
+  for (int i = 0; i < capacity && i < *nsteps; ++i) {
+    if (ms_total) ms_total[i] = ex->prof_ms[static_cast<size_t>(i)];
+    if (calls) calls[i] = ex->prof_calls[static_cast<size_t>(i)];
+    if (bytes) bytes[i] = ex->steps[static_cast<size_t>(i)].bytes_moved;
+  }
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" const char *flagfftxStepLabel(flagfftHandle handle, int direction, int step) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return nullptr;
+  std::lock_guard<std::mutex> lk(p->mu);
+  flagfft::Executable *ex = flagfft::exec_for(p, direction);
+  if (ex == nullptr || step < 0 || step >= static_cast<int>(ex->steps.size())) return nullptr;
+  return ex->steps[static_cast<size_t>(step)].label.c_str();
+}
+
 extern "C" flagfftResult flagfftxSetTuning(const flagfftxTuning *t) {
   flagfft::PlanOptions o;
   if (t != nullptr) {

@@ -32,16 +32,74 @@ void Executable::allocate_workspaces() {
   for (size_t b : ws_bytes) ws.push_back(std::make_shared<DevBuf>(std::max<size_t>(b, 256)));
 }
 
+Executable::~Executable() {
+  for (auto &p : ev_pending_) {
+    (void)hipEventDestroy(p.second.start);
+    (void)hipEventDestroy(p.second.stop);
+  }
+  for (auto &p : ev_free_) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+}
+
+void Executable::set_profiling(bool on) {
+  (void)harvest_profile();
+  profiling = on;
+  prof_ms.assign(steps.size(), 0.0);
+  prof_calls.assign(steps.size(), 0);
+}
+
+hipError_t Executable::take_pair(EvPair &p) const {
+  if (!ev_free_.empty()) {
+    p = ev_free_.back();
+    ev_free_.pop_back();
+    return hipSuccess;
+  }
+  hipError_t e = hipEventCreate(&p.start);
+  if (e != hipSuccess) return e;
+  return hipEventCreate(&p.stop);
+}
+
+hipError_t Executable::harvest_profile() const {
+  hipError_t first_err = hipSuccess;
+  if (prof_ms.size() != steps.size()) {
+    prof_ms.assign(steps.size(), 0.0);
+    prof_calls.assign(steps.size(), 0);
+  }
+  for (auto &pe : ev_pending_) {
+    hipError_t e = hipEventSynchronize(pe.second.stop);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, pe.second.start, pe.second.stop);
+    if (e == hipSuccess) {
+      prof_ms[pe.first] += ms;
+      prof_calls[pe.first] += 1;
+    } else if (first_err == hipSuccess) {
+      first_err = e;
+    }
+    ev_free_.push_back(pe.second);
+  }
+  ev_pending_.clear();
+  return first_err;
+}
+
 hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
   auto resolve = [&](int id) -> void * {
     if (id == kBufIn) return const_cast<void *>(in);
     if (id == kBufOut) return out;
     return ws[static_cast<size_t>(id - kBufWs0)]->ptr;
   };
-  for (const Step &s : steps) {
+  for (size_t si = 0; si < steps.size(); ++si) {
+    const Step &s = steps[si];
     void *src = resolve(s.src);
     void *dst = resolve(s.dst);
     hipError_t e = hipSuccess;
+    EvPair ev;
+    if (profiling) {
+      e = take_pair(ev);
+      if (e == hipSuccess) e = hipEventRecord(ev.start, st);
+      if (e != hipSuccess) return e;
+    }
     switch (s.kind) {
       case Step::Pass: {
         PassArgs a = s.pa;
@@ -73,6 +131,11 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
       }
     }
     if (e != hipSuccess) return e;
+    if (profiling) {
+      e = hipEventRecord(ev.stop, st);
+      if (e != hipSuccess) return e;
+      ev_pending_.emplace_back(si, ev);
+    }
   }
   return hipSuccess;
 }
@@ -99,6 +162,24 @@ int64_t unit_stride(bool col_fast, int64_t tcnt, int64_t ncol, const Geometry &g
 bool slot_fast(int64_t elem_stride, int64_t unit_str) { return elem_stride != 1 && unit_str == 1; }
 
 int odd_stride(int ext) { return (ext % 2 == 0) ? ext + 1 : ext; }
+
+// Algorithmic bytes read from / written to HBM for one sequence of a pass.
+int64_t src_bytes(int mode, int64_t seq_len, int64_t nlog, int eb) {
+  const int bm = mode & 3;
+  const int64_t elems = (mode & kSrcChirpFlag) ? nlog : seq_len;
+  if (bm == kSrcReal) return elems * eb / 2;
+  if (bm == kSrcHermitian) return (nlog / 2 + 1) * eb;
+  return elems * eb;
+}
+
+int64_t dst_bytes(int mode, int64_t seq_len, int64_t nlog, int eb) {
+  if (mode & kDstPointwiseFlag) return seq_len * eb;
+  const int bm = mode & 3;
+  const int64_t elems = (mode & kDstChirpFlag) ? nlog : seq_len;
+  if (bm == kDstHalf) return (nlog / 2 + 1) * eb;
+  if (bm == kDstReal) return elems * eb / 2;
+  return elems * eb;
+}
 
 }  // namespace
 
@@ -325,7 +406,7 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
   a.xcd_remap = 0;
   st.ept = sh.ept;
   st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-  st.bytes_moved = a.units * R * eb() * 2;
+  st.bytes_moved = a.units * (src_bytes(a.src_mode, R, a.nlog, eb()) + dst_bytes(a.dst_mode, R, a.nlog, eb()));
   std::ostringstream lab;
   lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> single-pass R=" << R << " C=" << sh.C
       << " tpu=" << sh.tpu << " units=" << a.units;
@@ -419,7 +500,8 @@ void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, c
     a.xcd_remap = opt_.xcd_remap;
     st.ept = sh.ept;
     st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-    st.bytes_moved = a.units * R * eb() * 2;
+    st.bytes_moved = s.nbatch * s.ncol *
+                     (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
     std::ostringstream lab;
     lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> stockham-pass " << j << "/" << k << " R=" << R
         << " p=" << p << " tcnt=" << tcnt << " C=" << sh.C << " tpu=" << sh.tpu;
@@ -474,7 +556,7 @@ void Builder::emit_rader(const Core &c, const SeqSet &s, int src, int dst, const
   const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
   st.ept = sh.ept;
   st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-  st.bytes_moved = a.nseqs * p * eb() * 2;
+  st.bytes_moved = a.nseqs * (src_bytes(a.src_mode, p, p, eb()) + dst_bytes(a.dst_mode, p, p, eb()));
   std::ostringstream lab;
   lab << "k_rader<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> p=" << p << " m=" << mm << " C=" << sh.C
       << " tpu=" << sh.tpu;
@@ -551,7 +633,7 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
   const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
   st.ept = sh.ept;
   st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-  st.bytes_moved = a.nseqs * n * eb() * 2;
+  st.bytes_moved = a.nseqs * (src_bytes(a.src_mode, n, n, eb()) + dst_bytes(a.dst_mode, n, n, eb()));
   std::ostringstream lab;
   lab << "k_bluestein<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> n=" << n << " M=" << M << " C=" << sh.C
       << " tpu=" << sh.tpu;
@@ -634,7 +716,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     const int64_t nblocks = (a.nseqs + sh.C - 1) / sh.C;
     st.ept = sh.ept;
     st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-    st.bytes_moved = a.nseqs * (n * eb() / 2 + (h + 1) * eb());
+    st.bytes_moved = a.nseqs * (n * eb() / 2 + (h + 1) * eb());  // n reals one way, h+1 bins the other
     std::ostringstream lab;
     lab << (pl.real_fwd ? "k_r2c_even<" : "k_c2r_even<") << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> n=" << n
         << " half=" << h << " C=" << sh.C << " tpu=" << sh.tpu;
@@ -664,6 +746,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     st.ra.ncol = static_cast<int32_t>(s.ncol);
     st.ra.n = static_cast<int32_t>(n);
     st.ra.tw_half = tw_half;
+    st.bytes_moved = st.ra.nseqs * (h + (h + 1)) * eb();
     st.label = "k_real_post n=" + std::to_string(n);
     ex_.steps.push_back(st);
   } else {
@@ -679,6 +762,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     st.ra.ncol = static_cast<int32_t>(s.ncol);
     st.ra.n = static_cast<int32_t>(n);
     st.ra.tw_half = tw_half;
+    st.bytes_moved = st.ra.nseqs * ((h + 1) + h) * eb();
     st.label = "k_real_pre n=" + std::to_string(n);
     ex_.steps.push_back(st);
     SeqSet sc = s;

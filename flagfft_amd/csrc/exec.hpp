@@ -58,10 +58,30 @@ class Executable {
   std::vector<std::shared_ptr<DevBuf>> ws;
   std::map<std::string, std::shared_ptr<DevBuf>> tables;
 
+  Executable() = default;
+  ~Executable();
+  Executable(const Executable &) = delete;
+  Executable &operator=(const Executable &) = delete;
+
   // Enqueue every step on `st`; in/out are device pointers.
   hipError_t run(const void *in, void *out, hipStream_t st) const;
   std::string describe() const;
   void allocate_workspaces();
+
+  // Per-launch HIP-event timing (flagfftxSetProfiling / flagfftxGetProfile).
+  void set_profiling(bool on);
+  hipError_t harvest_profile() const;  // waits for recorded events, accumulates
+  bool profiling = false;
+  mutable std::vector<double> prof_ms;
+  mutable std::vector<int64_t> prof_calls;
+
+ private:
+  struct EvPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+  };
+  mutable std::vector<EvPair> ev_free_;
+  mutable std::vector<std::pair<size_t, EvPair>> ev_pending_;
+  hipError_t take_pair(EvPair &p) const;
 };
 
 // Geometry of a batch of equal-length sequences in one buffer.

@@ -36,9 +36,11 @@ __device__ __forceinline__ int remap_block(int b, int nblocks, int enable) {
   return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
 
+// Logical input element e: base source, then conjugation (inverse by the
+// conj-forward-conj identity), then the Bluestein chirp.
 template <typename T>
 __device__ __forceinline__ V2<T> src_elem(const void *in, int64_t base, int64_t es, int mode, int64_t n,
-                                          const V2<T> *chirp, int64_t e) {
+                                          const V2<T> *chirp, int conj_in, int64_t e) {
   if ((mode & kSrcChirpFlag) && e >= n) return V2<T>{0, 0};
   const int bm = mode & 3;
   V2<T> v;
@@ -57,6 +59,7 @@ __device__ __forceinline__ V2<T> src_elem(const void *in, int64_t base, int64_t 
       v.y = -v.y;
     }
   }
+  if (conj_in) v.y = -v.y;
   if (mode & kSrcChirpFlag) v = cmul(v, chirp[e]);
   return v;
 }
@@ -148,6 +151,7 @@ __device__ __forceinline__ void lds_fft(V2<T> *buf, int R, int nst, const int32_
       case 11: lds_stage<T, 11, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       case 12: lds_stage<T, 12, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       case 13: lds_stage<T, 13, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
+      case 14: lds_stage<T, 14, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       case 15: lds_stage<T, 15, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       case 16: lds_stage<T, 16, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       default: break;  // the planner never emits other radices
@@ -202,8 +206,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
       }
       const int64_t base = static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs;
       const int64_t elog = static_cast<int64_t>(i) + static_cast<int64_t>(e) * a.tcnt;
-      v = src_elem<T>(a.in, base, a.gi.es, a.src_mode, a.nlog, chirp, elog);
-      if (a.conj_in) v.y = -v.y;
+      v = src_elem<T>(a.in, base, a.gi.es, a.src_mode, a.nlog, chirp, a.conj_in, elog);
       if (a.p > 1 && e > 0) {
         const uint32_t k = i - fdiv(i, a.fd_p) * a.p;
         const uint32_t mm = static_cast<uint32_t>(e) * k * static_cast<uint32_t>(a.twp_mult);
@@ -296,8 +299,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
     const int64_t q = q0 + s;
     V2<T> v{0, 0};
     if (q < a.nseqs) {
-      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode, p, nullptr, e);
-      if (a.conj_in) v.y = -v.y;
+      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode, p, nullptr, a.conj_in,
+                      e);
     }
     lds[s * LS + xoff + lpad(e)] = v;
   }
@@ -359,8 +362,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs 
     const int64_t q = q0 + ss;
     V2<T> v{0, 0};
     if (q < a.nseqs) {
-      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode & 3, n, nullptr, e);
-      if (a.conj_in) v.y = -v.y;
+      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode & 3, n, nullptr,
+                      a.conj_in, e);
       v = cmul(v, chirp[e]);
     }
     lds[ss * LS + lpad(e)] = v;

@@ -51,7 +51,8 @@ SYMBOLS = [
     "flagfftExecC2C", "flagfftExecZ2Z", "flagfftExecR2C", "flagfftExecD2Z", "flagfftExecC2R", "flagfftExecZ2D",
     "flagfftSetStream", "flagfftDestroy", "flagfftGetPlanDescription",
 ]
-EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning"]
+EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning",
+               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel"]
 
 
 class FlagFFTError(RuntimeError):
@@ -103,6 +104,9 @@ def lib():
             "flagfftxPlanDescribe": [_int, _IP, _int, _int, ctypes.c_char_p, ctypes.c_size_t],
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
             "flagfftxSetTuning": [ctypes.POINTER(Tuning)],
+            "flagfftxSetProfiling": [_H, _int],
+            "flagfftxGetProfile": [_H, _int, _int, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _IP],
         }
         for name, argtypes in sigs.items():
             fn = getattr(L, name)
@@ -110,6 +114,8 @@ def lib():
             fn.restype = _int
         L.flagfftGetPlanDescription.argtypes = [_H]
         L.flagfftGetPlanDescription.restype = ctypes.c_char_p
+        L.flagfftxStepLabel.argtypes = [_H, _int, _int]
+        L.flagfftxStepLabel.restype = ctypes.c_char_p
         L.flagfftxVersion.argtypes = []
         L.flagfftxVersion.restype = ctypes.c_char_p
         _lib = L
@@ -216,6 +222,28 @@ def set_tuning(**kw) -> int:
         else:
             setattr(t, k, v)
     return lib().flagfftxSetTuning(ctypes.byref(t))
+
+
+def set_profiling(h, enable: bool) -> int:
+    return lib().flagfftxSetProfiling(h, 1 if enable else 0)
+
+
+def get_profile(h, direction: int = FORWARD):
+    """Per-launch [(label, total_ms, calls, bytes)] of one Exec direction (waits for the events)."""
+    n = ctypes.c_int(0)
+    r = lib().flagfftxGetProfile(h, direction, 0, None, None, None, ctypes.byref(n))
+    check("flagfftxGetProfile", r)
+    cap = n.value
+    ms = (ctypes.c_double * max(cap, 1))()
+    calls = (ctypes.c_longlong * max(cap, 1))()
+    nbytes = (ctypes.c_longlong * max(cap, 1))()
+    r = lib().flagfftxGetProfile(h, direction, cap, ms, calls, nbytes, ctypes.byref(n))
+    check("flagfftxGetProfile", r)
+    out = []
+    for i in range(cap):
+        label = lib().flagfftxStepLabel(h, direction, i)
+        out.append((label.decode() if label else "", ms[i], calls[i], nbytes[i]))
+    return out
 
 
 def version() -> str:

@@ -29,6 +29,21 @@ flagfftResult flagfftxPlanDescribe(int rank, const int *n, flagfftType type, int
  * their staged HBM bytes (read + write of every pass, workspaces included). */
 flagfftResult flagfftxPlanStats(flagfftHandle plan, int direction, int *launches, long long *pass_bytes);
 
+/* Per-launch HIP-event timing of a plan's kernels, recorded on the plan's
+ * stream around every launch while enabled; enabling or disabling resets the
+ * accumulated totals. */
+flagfftResult flagfftxSetProfiling(flagfftHandle plan, int enable);
+
+/* Waits for the recorded events and reports, for each launch of one Exec of
+ * `direction`: accumulated milliseconds, number of timed calls, and the
+ * launch's algorithmic HBM bytes (read + write).  *nsteps receives the launch
+ * count; at most `capacity` entries are written to each array. */
+flagfftResult flagfftxGetProfile(flagfftHandle plan, int direction, int capacity, double *ms_total, long long *calls,
+                                 long long *bytes, int *nsteps);
+
+/* Label of launch `step` of `direction` (kernel family, shape); NULL when out of range. */
+const char *flagfftxStepLabel(flagfftHandle plan, int direction, int step);
+
 /* Plan-time tunables for plans created afterwards by this thread (0 keeps
  * the default).  Used by the autotuner to sweep decompositions. */
 typedef struct flagfftxTuning_t {

@@ -1,0 +1,117 @@
+"""2D transforms against the oracle (GPU).
+
+Mirrors ctest/test_2d_correctness.cpp: its sizes {16^2, 64^2, 128x256, 256x128,
+256^2} and non-smooth {23^2, 997x16, 16x997}, batch {1, 4} via PlanMany,
+forward/inverse roundtrip, R2C/C2R, plus the 4096^2 R2C/C2R and 2048^2 Z2Z configs.
+"""
+import numpy as np
+import pytest
+
+from flagfft_amd import flagfft as ff
+from oracle import oracle as ora
+
+from .helpers import check_accuracy, complex_input, dev_tensor, host, real_input, stream_of, sync
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16), (64, 64), (128, 256), (256, 128), (256, 256), (23, 23), (997, 16), (16, 997), (67, 32)]
+
+
+def _plan(n0, n1, t, batch):
+    if batch == 1:
+        return ff.Plan.two_d(n0, n1, t)
+    real_fwd = t in (ff.Type.R2C, ff.Type.D2Z)
+    real_inv = t in (ff.Type.C2R, ff.Type.Z2D)
+    h = n1 // 2 + 1
+    idist = n0 * h if real_inv else n0 * n1
+    odist = n0 * h if real_fwd else n0 * n1
+    return ff.Plan.many(2, [n0, n1], None, 1, idist, None, 1, odist, t, batch)
+
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_c2c_2d(gpu, shape, dbl):
+    n0, n1 = shape
+    t = ff.Type.Z2Z if dbl else ff.Type.C2C
+    for batch in (1, 4):
+        x = complex_input(n0 * 7919 + n1 + batch, (batch, n0, n1), np.complex128 if dbl else np.complex64)
+        xd = dev_tensor(x, gpu)
+        yd = xd.clone().zero_()
+        zd = xd.clone().zero_()
+        with _plan(n0, n1, t, batch) as p:
+            p.set_stream(stream_of(gpu))
+            ex = p.z2z if dbl else p.c2c
+            ex(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
+            ex(yd.data_ptr(), zd.data_ptr(), ff.INVERSE)
+            sync(gpu)
+        ref = ora.fft2(x.astype(np.complex128))
+        n = n0 * n1
+        check_accuracy(host(yd), ref, t, n, row_len=n, factor=2.0, what=f"2D fwd batch={batch}")
+        check_accuracy(host(zd), x.astype(np.complex128) * n, t, n, row_len=n, factor=4.0, what="2D roundtrip")
+
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("shape", SHAPES + [(33, 15)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_real_2d(gpu, shape, dbl):
+    n0, n1 = shape
+    h = n1 // 2 + 1
+    tf = ff.Type.D2Z if dbl else ff.Type.R2C
+    ti = ff.Type.Z2D if dbl else ff.Type.C2R
+    for batch in (1, 2):
+        x = real_input(n0 + 31 * n1 + batch, (batch, n0, n1), np.float64 if dbl else np.float32)
+        xd = dev_tensor(x, gpu)
+        Xd = dev_tensor(np.zeros((batch, n0, h), np.complex128 if dbl else np.complex64), gpu)
+        zd = xd.clone().zero_()
+        with _plan(n0, n1, tf, batch) as pf, _plan(n0, n1, ti, batch) as pi:
+            pf.set_stream(stream_of(gpu))
+            pi.set_stream(stream_of(gpu))
+            (pf.d2z if dbl else pf.r2c)(xd.data_ptr(), Xd.data_ptr())
+            (pi.z2d if dbl else pi.c2r)(Xd.data_ptr(), zd.data_ptr())
+            sync(gpu)
+        n = n0 * n1
+        check_accuracy(host(Xd), ora.rfft2(x.astype(np.float64)), tf, n, row_len=n0 * h, factor=2.0,
+                       what=f"2D R2C batch={batch}")
+        check_accuracy(host(zd), x.astype(np.float64) * n, ti, n, row_len=n, factor=4.0, what="2D roundtrip")
+
+
+def test_c2r_2d_against_oracle(gpu):
+    """2D C2R of a non-Hermitian input: the row pass drops Im of each row's DC/Nyquist bins."""
+    n0, n1 = 24, 20
+    h = n1 // 2 + 1
+    X = complex_input(99, (1, n0, h), np.complex64)
+    Xd = dev_tensor(X, gpu)
+    yd = dev_tensor(np.zeros((1, n0, n1), np.float32), gpu)
+    with ff.Plan.two_d(n0, n1, ff.Type.C2R) as p:
+        p.set_stream(stream_of(gpu))
+        p.c2r(Xd.data_ptr(), yd.data_ptr())
+        sync(gpu)
+    check_accuracy(host(yd), ora.irfft2(X.astype(np.complex128), n1), ff.Type.C2R, n0 * n1, row_len=n0 * n1,
+                   factor=2.0)
+
+
+def test_config_2d(gpu):
+    """2D R2C/C2R FP32 4096x4096 (N(0,1)) and 2D Z2Z FP64 2048x2048."""
+    n0 = n1 = 4096
+    h = n1 // 2 + 1
+    x = real_input(0x2D, (1, n0, n1), np.float32, kind="normal")
+    xd = dev_tensor(x, gpu)
+    Xd = dev_tensor(np.zeros((1, n0, h), np.complex64), gpu)
+    zd = xd.clone().zero_()
+    with ff.Plan.two_d(n0, n1, ff.Type.R2C) as pf, ff.Plan.two_d(n0, n1, ff.Type.C2R) as pi:
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        pf.r2c(xd.data_ptr(), Xd.data_ptr())
+        pi.c2r(Xd.data_ptr(), zd.data_ptr())
+        sync(gpu)
+    n = n0 * n1
+    check_accuracy(host(Xd), ora.rfft2(x.astype(np.float64)), ff.Type.R2C, n, row_len=n0 * h, what="4096^2 R2C")
+    check_accuracy(host(zd), x.astype(np.float64) * n, ff.Type.C2R, n, row_len=n, factor=2.0, what="4096^2 rt")
+    n0 = n1 = 2048
+    x = complex_input(0x2E, (1, n0, n1), np.complex128, kind="normal")
+    xd = dev_tensor(x, gpu)
+    yd = xd.clone().zero_()
+    with ff.Plan.two_d(n0, n1, ff.Type.Z2Z) as p:
+        p.set_stream(stream_of(gpu))
+        p.z2z(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
+        sync(gpu)
+    check_accuracy(host(yd), ora.fft2(x), ff.Type.Z2Z, n0 * n1, row_len=n0 * n1, what="2048^2 Z2Z")
