@@ -1,0 +1,308 @@
+"""Benchmark of the FFT execution path behind include/flagfft.h on one MI355X.
+
+Default workload (arbitrary-length 1D, one "step" = one pass of the hot path
+over one batch of synthetic input):
+  C2C f32 and Z2Z f64 forward at N=10007 (prime) and N=10935 = 3^7*5, batch 64, N(0,1);
+  R2C->C2R f32 and D2Z->Z2D f64 at N=6000, batch 128, U[-1,1].
+Every transform goes through the C ABI (libflagfft.so) on one HIP stream.
+Inputs rotate over buffer sets larger than 2x the 256 MB Infinity Cache so
+the timed region reads HBM.
+
+value          algorithmic HBM GB/s of the whole step: sum over transforms of
+               batch * (input bytes + output bytes), divided by wall time
+roofline       dominant kernel (by total device time): algorithmic bytes per
+               launch / mean launch duration, both from HIP events recorded on
+               the plan's stream around every launch inside the timed region
+cpu_baseline   the oracle's fast FFT (oracle/oracle.c) on the host cores over
+               a bounded sample of the same step
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--suite]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "C2C/R2C effective HBM GB/s (% of peak); mixed-length serving transforms/s + p99"
+HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Case:
+    """One transform family of the workload: plans, rotating device buffers, one exec."""
+
+    def __init__(self, name, kind, n, batch, dbl, dev, ff, torch):
+        self.name, self.kind, self.n, self.batch, self.dbl = name, kind, n, batch, dbl
+        self.ff, self.torch, self.dev = ff, torch, dev
+        cdt = torch.complex128 if dbl else torch.complex64
+        rdt = torch.float64 if dbl else torch.float32
+        esz = 16 if dbl else 8
+        h = n // 2 + 1
+        T = ff.Type
+        if kind == "c2c":
+            self.plans = [ff.Plan.one_d(n, T.Z2Z if dbl else T.C2C, batch)]
+            self.alg_bytes = batch * n * esz * 2
+            self.shapes = [((batch, n), cdt), ((batch, n), cdt)]
+        else:  # real roundtrip R2C -> C2R
+            self.plans = [ff.Plan.one_d(n, T.D2Z if dbl else T.R2C, batch),
+                          ff.Plan.one_d(n, T.Z2D if dbl else T.C2R, batch)]
+            self.alg_bytes = 2 * batch * (n * esz // 2 + h * esz)
+            self.shapes = [((batch, n), rdt), ((batch, h), cdt), ((batch, n), rdt)]
+        self.sets = []
+
+    def footprint(self):
+        return sum(int(np.prod(s)) * self.torch.empty((), dtype=d).element_size() for s, d in self.shapes)
+
+    def allocate(self, nsets, seed):
+        torch = self.torch
+        g = torch.Generator(device=self.dev)
+        g.manual_seed(seed)
+        for _ in range(nsets):
+            bufs = []
+            for i, (shape, dt) in enumerate(self.shapes):
+                if i == 0:
+                    if dt.is_complex:
+                        t = torch.randn(shape, dtype=dt, device=self.dev, generator=g)  # re, im iid N(0, 1/2)
+                        t = t * (2.0 ** 0.5)                                             # -> iid N(0, 1)
+                    else:
+                        t = torch.rand(shape, dtype=dt, device=self.dev, generator=g) * 2 - 1  # U[-1, 1)
+                else:
+                    t = torch.empty(shape, dtype=dt, device=self.dev)
+                bufs.append(t)
+            self.sets.append(bufs)
+
+    def bind(self, stream):
+        for p in self.plans:
+            p.set_stream(stream)
+
+    def run(self, step):
+        b = self.sets[step % len(self.sets)]
+        p = self.plans
+        if self.kind == "c2c":
+            (p[0].z2z if self.dbl else p[0].c2c)(b[0].data_ptr(), b[1].data_ptr(), self.ff.FORWARD)
+        else:
+            (p[0].d2z if self.dbl else p[0].r2c)(b[0].data_ptr(), b[1].data_ptr())
+            (p[1].z2d if self.dbl else p[1].c2r)(b[1].data_ptr(), b[2].data_ptr())
+
+    def cpu_step(self, ora, nthreads, frac):
+        """Oracle fast FFT over a fraction of the batch; returns (seconds, algorithmic bytes)."""
+        nb = max(1, int(self.batch * frac))
+        rng = np.random.default_rng(self.n)
+        if self.kind == "c2c":
+            x = (rng.standard_normal((nb, self.n)) + 1j * rng.standard_normal((nb, self.n)))
+            x = x.astype(np.complex128 if self.dbl else np.complex64)
+            t0 = time.perf_counter()
+            (ora.fft if self.dbl else ora.fft_f32)(x, -1, nthreads)
+            dt = time.perf_counter() - t0
+        else:
+            x = rng.uniform(-1, 1, (nb, self.n)).astype(np.float64 if self.dbl else np.float32)
+            t0 = time.perf_counter()
+            X = (ora.rfft if self.dbl else ora.rfft_f32)(x, nthreads)
+            (ora.irfft if self.dbl else ora.irfft_f32)(X, self.n, nthreads)
+            dt = time.perf_counter() - t0
+        return dt, self.alg_bytes * nb / self.batch
+
+
+WORKLOAD = [
+    ("C2C f32 N=10007 x64", "c2c", 10007, 64, False),
+    ("C2C f32 N=10935 x64", "c2c", 10935, 64, False),
+    ("Z2Z f64 N=10007 x64", "c2c", 10007, 64, True),
+    ("Z2Z f64 N=10935 x64", "c2c", 10935, 64, True),
+    ("R2C->C2R f32 N=6000 x128", "real", 6000, 128, False),
+    ("D2Z->Z2D f64 N=6000 x128", "real", 6000, 128, True),
+]
+
+SUITE = [  # the other configurations, reported with --suite (not the headline line)
+    [("C2C f32 N=4096 x256", "c2c", 4096, 256, False)],
+    [("C2C f32 N=2^20 x16", "c2c", 1 << 20, 16, False)],
+    [("C2C f32 N=2^24 x1", "c2c", 1 << 24, 1, False)],
+    [("C2C f32 N=1024 x65536", "c2c", 1024, 65536, False)],
+    [("R2C->C2R f32 N=2^20 x16", "real", 1 << 20, 16, False)],
+]
+
+
+def kernel_symbol(label: str) -> str:
+    """Step label 'k_pass<fp32,16> ...' -> the kernel symbol rocprof reports."""
+    head = label.split(" ")[0]
+    fam, _, targs = head.partition("<")
+    if not targs:
+        return fam
+    prec, ept = targs.rstrip(">").split(",")
+    return f"{fam}<{'float' if prec == 'fp32' else 'double'}, {ept}>"
+
+
+def measure(cases, steps, warmup, torch, ff, stream):
+    for c in cases:
+        c.bind(stream.cuda_stream)
+        for p in c.plans:
+            ff.check("flagfftxSetProfiling", ff.set_profiling(p.handle, True))
+    with torch.cuda.stream(stream):
+        for s in range(warmup):
+            for c in cases:
+                c.run(s)
+        torch.cuda.synchronize()
+        for c in cases:  # restart the per-launch totals: the timed region only
+            for p in c.plans:
+                ff.set_profiling(p.handle, True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            for c in cases:
+                c.run(warmup + s)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    launches = {}
+    per_case = {}
+    for c in cases:
+        case_ms = 0.0
+        for pi, p in enumerate(c.plans):
+            direction = ff.FORWARD if (c.kind == "c2c" or pi == 0) else ff.INVERSE
+            for label, ms, calls, nbytes in ff.get_profile(p.handle, direction):
+                key = (c.name, pi, label)
+                launches[key] = (ms, calls, nbytes)
+                case_ms += ms
+        per_case[c.name] = {"alg_bytes": c.alg_bytes, "device_ms_per_exec": case_ms / steps,
+                            "gbps": c.alg_bytes / (case_ms / steps * 1e-3) / 1e9 if case_ms > 0 else None}
+    return (t1 - t0), launches, per_case
+
+
+def roofline_of(launches):
+    """Dominant kernel symbol by total device time; achieved = bytes / mean duration."""
+    by_sym = {}
+    for (case, pi, label), (ms, calls, nbytes) in launches.items():
+        sym = kernel_symbol(label)
+        d = by_sym.setdefault(sym, {"ms": 0.0, "calls": 0, "bytes": 0, "labels": set()})
+        d["ms"] += ms
+        d["calls"] += calls
+        d["bytes"] += nbytes * calls
+        d["labels"].add(f"{case}: {label}")
+    sym, d = max(by_sym.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = d["ms"] / d["calls"]
+    bytes_per_launch = d["bytes"] / d["calls"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": sym, "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": int(bytes_per_launch),
+        "launches": d["calls"], "share_of_device_time": round(d["ms"] / sum(v["ms"] for v in by_sym.values()), 3),
+    }, {k: {"ms": round(v["ms"], 3), "calls": v["calls"], "avg_us": round(v["ms"] / max(v["calls"], 1) * 1e3, 2)}
+        for k, v in by_sym.items()}
+
+
+def cpu_baseline(cases, budget_s=4.0):
+    from oracle import oracle as ora
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    nthreads = max(1, min(nthreads, os.cpu_count() or 1))
+    frac = 0.25
+    total_t = 0.0
+    total_b = 0.0
+    reps = 0
+    t_start = time.perf_counter()
+    while reps < 1 or time.perf_counter() - t_start < budget_s:
+        for c in cases:
+            dt, b = c.cpu_step(ora, nthreads, frac)
+            total_t += dt
+            total_b += b
+        reps += 1
+        if reps >= 20:
+            break
+    return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "sample": f"{reps} x the same step with each batch cut to {int(frac * 100)}% "
+                      f"(oracle fast FFT, FP32 for f32 cases, FP64 for f64 cases, {total_t:.1f} s wall)"}
+
+
+def run_workload(spec, args, torch, ff, dev, stream, seed):
+    cases = [Case(name, kind, n, batch, dbl, dev, ff, torch) for name, kind, n, batch, dbl in spec]
+    footprint = sum(c.footprint() for c in cases)
+    nsets = max(1, -(-2 * MALL_BYTES // max(footprint, 1)) + 1)
+    nsets = min(nsets, 64)
+    for i, c in enumerate(cases):
+        c.allocate(nsets, seed + i)
+    torch.cuda.synchronize()
+    t, launches, per_case = measure(cases, args.steps, args.warmup, torch, ff, stream)
+    alg = sum(c.alg_bytes for c in cases)
+    value = args.steps * alg / t / 1e9
+    roof, kernels = roofline_of(launches)
+    pass_bytes = sum(nb * calls for (ms, calls, nb) in launches.values())
+    return cases, {"value": value, "ms_per_step": t / args.steps * 1e3, "alg_bytes_per_step": alg,
+                   "pass_gbps": pass_bytes / t / 1e9, "roofline": roof, "kernels": kernels,
+                   "per_case": per_case, "buffer_sets": nsets, "footprint_mb": round(nsets * footprint / 2**20)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--suite", action="store_true", help="also measure the other configurations (stderr)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the path shards by batch with no communication; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from flagfft_amd import flagfft as ff
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    cases, res = run_workload(WORKLOAD, args, torch, ff, dev, stream, 1234)
+
+    cpu = None if args.no_cpu else cpu_baseline(cases)
+    line = {
+        "metric": METRIC,
+        "value": round(res["value"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "arbitrary-length 1D: C2C f32 + Z2Z f64 forward at N=10007 (prime) and N=10935=3^7*5, "
+                        "batch 64, N(0,1); R2C->C2R f32 + D2Z->Z2D f64 at N=6000, batch 128, U[-1,1]",
+            "transforms_per_step": sum(c.batch * (1 if c.kind == "c2c" else 2) for c in cases),
+            "alg_bytes_per_step": res["alg_bytes_per_step"],
+            "hbm_frac_of_peak": round(res["value"] / HBM_PEAK_GBS, 4),
+            "pass_bytes_gbps": round(res["pass_gbps"], 2),
+            "input_rotation": f"{res['buffer_sets']} buffer sets, {res['footprint_mb']} MiB",
+            "parallelism": "single GPU, one HIP stream",
+        },
+        "roofline": res["roofline"],
+        "cpu_baseline": cpu,
+        "per_case": {k: {"gbps": round(v["gbps"], 1) if v["gbps"] else None,
+                         "device_us": round(v["device_ms_per_exec"] * 1e3, 2)} for k, v in res["per_case"].items()},
+        "kernels": res["kernels"],
+    }
+    if args.suite:
+        for spec in SUITE:
+            del cases
+            torch.cuda.empty_cache()
+            cases, r = run_workload(spec, args, torch, ff, dev, stream, 99)
+            log(json.dumps({"suite": spec[0][0], "gbps": round(r["value"], 1),
+                            "frac": round(r["value"] / HBM_PEAK_GBS, 3), "us_per_exec": round(r["ms_per_step"] * 1e3, 2),
+                            "roofline": r["roofline"]}))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

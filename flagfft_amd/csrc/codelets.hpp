@@ -17,18 +17,14 @@
 namespace flagfft {
 namespace dev {
 
+// Interleaved complex value.  A plain aggregate (not HIP's vector type) so it
+// can be loaded member-wise through address-space-qualified pointers.
 template <typename T>
-struct Vec2;
-template <>
-struct Vec2<float> {
-  using type = float2;
-};
-template <>
-struct Vec2<double> {
-  using type = double2;
+struct alignas(2 * sizeof(T)) Cx {
+  T x, y;
 };
 template <typename T>
-using V2 = typename Vec2<T>::type;
+using V2 = Cx<T>;
 
 template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) {

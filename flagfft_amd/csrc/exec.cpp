@@ -108,6 +108,13 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
         e = launch_pass(a, s.dbl, s.ept, s.shape, st);
         break;
       }
+      case Step::Leaf: {
+        LeafArgs a = s.la;
+        a.in = src;
+        a.out = dst;
+        e = launch_leaf(a, s.dbl, s.leaf_id, s.shape, st);
+        break;
+      }
       case Step::Rader:
       case Step::Bluestein:
       case Step::R2CEven:
@@ -325,37 +332,100 @@ struct ShapeChoice {
 };
 
 ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, bool chunked, int64_t nunits) {
-  ShapeChoice best;
-  bool found = false;
+  // Workgroups wanted in flight: two per CU of the 256-CU part.
+  constexpr int64_t kTargetGroups = 512;
   const bool dbl = eb == 16;
-  const int epts[2] = {dbl ? 8 : 16, dbl ? 16 : 32};
-  for (int ept : epts) {
+  const int epts_f[3] = {8, 16, 32};
+  const int epts_d[2] = {8, 16};
+  const int *epts = dbl ? epts_d : epts_f;
+  const int nept = dbl ? 2 : 3;
+  const int ept_default = dbl ? 8 : 16;
+  struct Cand {
+    ShapeChoice s;
+    int64_t groups;
+  };
+  std::vector<Cand> cands;
+  for (int i = 0; i < nept; ++i) {
+    const int ept = epts[i];
     if (o.ept && ept != o.ept) continue;
     int tpu = static_cast<int>((work_len + ept - 1) / ept);
     if (tpu < 1) tpu = 1;
     if (tpu > kMaxBlockThreads) continue;
     int desired = chunked ? std::max(1, o.chunk_bytes / eb) : std::max(1, o.target_threads / tpu);
+    if (!chunked && (nunits + desired - 1) / desired < kTargetGroups)
+      desired = static_cast<int>(std::max<int64_t>(1, nunits / kTargetGroups));
     if (o.forced_units > 0) desired = o.forced_units;
     if (desired > nunits) desired = static_cast<int>(std::max<int64_t>(nunits, 1));
     const size_t cap = static_cast<size_t>(chunked ? o.lds_cap_bytes : o.lds_pref_bytes);
     int C = desired;
     while (C > 1 && (C * tpu > kMaxBlockThreads || static_cast<size_t>(C) * LS * eb > cap)) --C;
     if (static_cast<size_t>(C) * LS * eb > static_cast<size_t>(o.lds_cap_bytes)) continue;
-    if (!found || C > best.C) {
-      best.C = C;
-      best.tpu = tpu;
-      best.ept = ept;
-      best.lds = static_cast<size_t>(C) * LS * eb;
-      found = true;
+    ShapeChoice sc;
+    sc.C = C;
+    sc.tpu = tpu;
+    sc.ept = ept;
+    sc.lds = static_cast<size_t>(C) * LS * eb;
+    cands.push_back({sc, (nunits + C - 1) / C});
+  }
+  if (cands.empty()) throw std::runtime_error("no launch shape fits the LDS / block limits");
+  // Few workgroups: the most threads per transform wins; otherwise the default
+  // elements-per-thread with the largest tile.
+  const Cand *best = &cands[0];
+  for (const Cand &c : cands) {
+    const bool starved = c.groups < kTargetGroups;
+    if (starved) {
+      if (c.groups * c.s.C * c.s.tpu > best->groups * best->s.C * best->s.tpu) best = &c;
+    } else if (c.s.C > best->s.C || (c.s.C == best->s.C && c.s.ept == ept_default)) {
+      best = &c;
     }
   }
-  if (!found) throw std::runtime_error("no launch shape fits the LDS / block limits");
-  return best;
+  return best->s;
 }
 
 }  // namespace
 
 void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  // Compile-time specialised kernel for contiguous complex rows of a listed radix sequence.
+  const int leaf_id = leaf_kernel_id(c.radices);
+  if (leaf_id >= 0 && !s.col_fast && s.ncol == 1 && s.gi.es == 1 && s.go.es == 1 && m.src_mode == kSrcComplex &&
+      m.dst_mode == kDstComplex && c.n > 1) {
+    Step st;
+    st.kind = Step::Leaf;
+    st.dbl = dbl();
+    st.src = src;
+    st.dst = dst;
+    st.leaf_id = leaf_id;
+    LeafArgs &a = st.la;
+    const int R = static_cast<int>(c.n);
+    const int ept = leaf_kernel_ept(leaf_id);
+    const int tpu = R / ept;
+    const int LS = odd_stride(lds_pad(R - 1) + 1);
+    a.nseq = s.nbatch;
+    a.idist = s.gi.bs;
+    a.odist = s.go.bs;
+    a.tw = roots(R, R, 1);
+    a.conj_in = m.conj_in;
+    a.conj_out = m.conj_out;
+    a.scale = m.scale;
+    int C = std::max(1, opt_.target_threads / tpu);
+    if ((s.nbatch + C - 1) / C < 512) C = static_cast<int>(std::max<int64_t>(1, s.nbatch / 512));
+    if (opt_.forced_units > 0) C = opt_.forced_units;
+    while (C > 1 && (C * tpu > kMaxBlockThreads || static_cast<size_t>(C) * LS * eb() > static_cast<size_t>(opt_.lds_pref_bytes)))
+      --C;
+    a.C = C;
+    const int64_t nblocks = (s.nbatch + C - 1) / C;
+    st.ept = ept;
+    st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(C * tpu)),
+                           static_cast<size_t>(C) * LS * eb()};
+    st.bytes_moved = s.nbatch * R * eb() * 2;
+    std::ostringstream lab;
+    lab << "k_leaf<" << (dbl() ? "fp64" : "fp32") << "," << ept << ">[";
+    for (size_t i = 0; i < c.radices.size(); ++i) lab << (i ? "x" : "") << c.radices[i];
+    lab << "] R=" << R << " C=" << C << " tpu=" << tpu << " transforms=" << s.nbatch;
+    st.label = lab.str();
+    ex_.steps.push_back(st);
+    return;
+  }
   Step st;
   st.kind = Step::Pass;
   st.dbl = dbl();
@@ -404,11 +474,13 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
   const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
   a.nblocks = static_cast<int32_t>(nblocks);
   a.xcd_remap = 0;
+  a.direct = (!lsf && !ssf && !s.col_fast && s.gi.es == 1 && s.go.es == 1) ? 1 : 0;
   st.ept = sh.ept;
   st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
   st.bytes_moved = a.units * (src_bytes(a.src_mode, R, a.nlog, eb()) + dst_bytes(a.dst_mode, R, a.nlog, eb()));
   std::ostringstream lab;
-  lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> single-pass R=" << R << " C=" << sh.C
+  lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> single-pass" << (a.direct ? "/direct" : "/staged")
+      << " R=" << R << " C=" << sh.C
       << " tpu=" << sh.tpu << " units=" << a.units;
   st.label = lab.str();
   ex_.steps.push_back(st);
@@ -498,6 +570,7 @@ void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, c
     const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
     a.nblocks = static_cast<int32_t>(nblocks);
     a.xcd_remap = opt_.xcd_remap;
+    a.direct = 0;
     st.ept = sh.ept;
     st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
     st.bytes_moved = s.nbatch * s.ncol *

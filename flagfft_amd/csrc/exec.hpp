@@ -39,11 +39,13 @@ struct DevBuf {
 enum BufId : int { kBufIn = 0, kBufOut = 1, kBufWs0 = 2 };
 
 struct Step {
-  enum Kind { Pass, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre } kind = Pass;
+  enum Kind { Pass, Leaf, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre } kind = Pass;
   bool dbl = false;
   int ept = 16;
+  int leaf_id = -1;  // Leaf: entry of the specialised kernel table
   int src = kBufIn, dst = kBufOut;
   PassArgs pa{};
+  LeafArgs la{};
   FusedArgs fa{};
   RealPassArgs ra{};
   LaunchShape shape{};

@@ -85,6 +85,7 @@ struct PassArgs {
   const void *bhat;      // Bluestein / Rader FFT of the convolution kernel
   int32_t xcd_remap;     // map consecutive tiles to one XCD
   int32_t nblocks;
+  int32_t direct;        // single-pass contiguous sequences: first/last stage talk to HBM directly
 };
 
 // Fused single-kernel transforms that keep a whole sequence in LDS.
@@ -114,6 +115,18 @@ struct FusedArgs {
   int32_t src_mode, dst_mode;
   int32_t conj_in, conj_out;
   double scale;
+};
+
+// Compile-time specialised single-pass transform of contiguous complex rows.
+struct LeafArgs {
+  const void *in;
+  void *out;
+  int64_t nseq;          // transforms
+  int64_t idist, odist;  // complex elements between consecutive transforms
+  const void *tw;        // w_N^j, j < N
+  int32_t conj_in, conj_out;
+  double scale;
+  int32_t C;             // transforms per workgroup
 };
 
 // Split-spectrum pre/post pass for even-length real transforms that do not fit one LDS transform.

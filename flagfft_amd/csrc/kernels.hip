@@ -16,88 +16,66 @@
 #include <hip/hip_runtime.h>
 
 #include "codelets.hpp"
+#include "dev_common.hpp"
 #include "fft_args.hpp"
 #include "kernels.hpp"
 
 namespace flagfft {
 namespace dev {
 
-__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
-  uint32_t t = __umulhi(n, f.mul);
-  return (t + n) >> f.shift;
-}
-
-// LDS index padding: one spare slot every 32 complex elements.
-__device__ __forceinline__ int lpad(int e) { return e + (e >> 5); }
-
-__device__ __forceinline__ int remap_block(int b, int nblocks, int enable) {
-  if (!enable || (nblocks & 7) != 0) return b;
-  // blocks are dispatched round-robin over the 8 XCDs: keep consecutive tiles on one XCD
-  return (b & 7) * (nblocks >> 3) + (b >> 3);
-}
-
-// Logical input element e: base source, then conjugation (inverse by the
-// conj-forward-conj identity), then the Bluestein chirp.
-template <typename T>
-__device__ __forceinline__ V2<T> src_elem(const void *in, int64_t base, int64_t es, int mode, int64_t n,
-                                          const V2<T> *chirp, int conj_in, int64_t e) {
-  if ((mode & kSrcChirpFlag) && e >= n) return V2<T>{0, 0};
-  const int bm = mode & 3;
-  V2<T> v;
-  if (bm == kSrcComplex) {
-    v = static_cast<const V2<T> *>(in)[base + e * es];
-  } else if (bm == kSrcReal) {
-    v = V2<T>{static_cast<const T *>(in)[base + e * es], T(0)};
-  } else {
-    const V2<T> *X = static_cast<const V2<T> *>(in);
-    const int64_t h = n >> 1;
-    if (e <= h) {
-      v = X[base + e * es];
-      if (e == 0 || 2 * e == n) v.y = T(0);
-    } else {
-      v = X[base + (n - e) * es];
-      v.y = -v.y;
+// Batched staged copy: every thread issues U independent loads before it
+// commits any of them, so U global (or LDS) round trips overlap instead of
+// serialising one per loop iteration.
+template <typename T, int U, typename Load, typename Commit>
+__device__ __forceinline__ void batched(int total, int tid, int nthr, Load load, Commit commit) {
+  for (int f0 = tid; f0 < total; f0 += U * nthr) {
+    V2<T> v[U];
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int f = f0 + uu * nthr;
+      if (f < total) v[uu] = load(f);
+    }
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int f = f0 + uu * nthr;
+      if (f < total) commit(f, v[uu]);
     }
   }
-  if (conj_in) v.y = -v.y;
-  if (mode & kSrcChirpFlag) v = cmul(v, chirp[e]);
-  return v;
 }
 
+constexpr int kBatch = 8;
+
+// Where the first stage reads and the last stage writes when they talk to HBM directly.
+struct SrcDesc {
+  const void *in;
+  int64_t base, es, nlog;
+  const void *chirp;
+  int mode, conj, active;
+};
+
 template <typename T>
-__device__ __forceinline__ void dst_elem(void *out, int64_t base, int64_t es, int mode, int64_t n,
-                                         const V2<T> *chirp, const V2<T> *bhat, int conj_out, T scale, int64_t e,
-                                         V2<T> v) {
-  if (mode & kDstPointwiseFlag) {
-    v = cconj(cmul(v, bhat[e]));
-    static_cast<V2<T> *>(out)[base + e * es] = v;
-    return;
-  }
-  if (mode & kDstChirpFlag) {
-    if (e >= n) return;
-    v = cmul(cconj(v), chirp[e]);
-  }
-  if (conj_out) v.y = -v.y;
-  if (scale != T(1)) {
-    v.x *= scale;
-    v.y *= scale;
-  }
-  const int bm = mode & 3;
-  if (bm == kDstComplex) {
-    static_cast<V2<T> *>(out)[base + e * es] = v;
-  } else if (bm == kDstHalf) {
-    if (2 * e <= n) static_cast<V2<T> *>(out)[base + e * es] = v;
-  } else {
-    static_cast<T *>(out)[base + e * es] = v.x;
-  }
-}
+struct DstDesc {
+  void *out;
+  int64_t base, es, nlog;
+  const void *chirp, *bhat;
+  int mode, conj, active;
+  T scale;
+};
+
+#define FLAGFFT_SRC_PARAMS \
+  const void *s_in, int64_t s_base, int64_t s_es, int64_t s_nlog, const void *s_chirp, int s_mode, int s_conj, int s_active
+#define FLAGFFT_DST_PARAMS                                                                                        \
+  void *d_out, int64_t d_base, int64_t d_es, int64_t d_nlog, const void *d_chirp, const void *d_bhat, int d_mode, \
+      int d_conj, int d_active, T d_scale
+#define FLAGFFT_SRC_ARGS(S) S.in, S.base, S.es, S.nlog, S.chirp, S.mode, S.conj, S.active
+#define FLAGFFT_DST_ARGS(D) D.out, D.base, D.es, D.nlog, D.chirp, D.bhat, D.mode, D.conj, D.active, D.scale
 
 // One radix-RR Stockham stage over an LDS-resident sequence of length R.
 // Prior product q; butterfly b: k = b mod q, inputs buf[b + j*R/RR] twiddled by
 // w_R^{j k R/(q RR)}, outputs to buf[(b-k)RR + k + j q].
 template <typename T, int RR, int E>
-__device__ __noinline__ void lds_stage(V2<T> *buf, int R, int q, const FastDiv fdq, int twstep, int lt, int tpu,
-                                       const V2<T> *__restrict__ tw) {
+__device__ __noinline__ void lds_stage(LdsC<T> *buf, int R, int q, const FastDiv fdq, int twstep, int lt, int tpu,
+                                       const GlbC<T> *__restrict__ tw) {
   constexpr int MAXB = (E + RR - 1) / RR;
   const int nb = R / RR;
   V2<T> v[MAXB][RR];
@@ -106,12 +84,12 @@ __device__ __noinline__ void lds_stage(V2<T> *buf, int R, int q, const FastDiv f
     const int b = lt + m * tpu;
     if (b < nb) {
 #pragma unroll
-      for (int j = 0; j < RR; ++j) v[m][j] = buf[lpad(b + j * nb)];
+      for (int j = 0; j < RR; ++j) v[m][j] = ldc(buf + lpad(b + j * nb));
       if (q > 1) {
         const int k = b - static_cast<int>(fdiv(b, fdq)) * q;
         const int st = k * twstep;
 #pragma unroll
-        for (int j = 1; j < RR; ++j) v[m][j] = cmul(v[m][j], tw[j * st]);
+        for (int j = 1; j < RR; ++j) v[m][j] = cmul(v[m][j], ldg(tw + j * st));
       }
       dft<RR>(v[m]);
     }
@@ -124,138 +102,268 @@ __device__ __noinline__ void lds_stage(V2<T> *buf, int R, int q, const FastDiv f
       const int k = (q > 1) ? b - static_cast<int>(fdiv(b, fdq)) * q : 0;
       const int base = (b - k) * RR + k;
 #pragma unroll
-      for (int j = 0; j < RR; ++j) buf[lpad(base + j * q)] = v[m][j];
+      for (int j = 0; j < RR; ++j) stc(buf + lpad(base + j * q), v[m][j]);
     }
   }
   __syncthreads();
 }
 
+// First stage (q = 1) reading its butterfly inputs straight from HBM.
+template <typename T, int RR, int E>
+__device__ __noinline__ void stage_first(LdsC<T> *buf, int R, FLAGFFT_SRC_PARAMS, int lt, int tpu) {
+  constexpr int MAXB = (E + RR - 1) / RR;
+  const int nb = R / RR;
+  V2<T> v[MAXB][RR];
+#pragma unroll
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = lt + m * tpu;
+    if (b < nb) {
+#pragma unroll
+      for (int j = 0; j < RR; ++j)
+        v[m][j] = s_active ? src_elem<T>(s_in, s_base, s_es, s_mode, s_nlog, s_chirp, s_conj, b + j * nb)
+                           : V2<T>{0, 0};
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = lt + m * tpu;
+    if (b < nb) {
+      dft<RR>(v[m]);
+#pragma unroll
+      for (int j = 0; j < RR; ++j) stc(buf + lpad(b * RR + j), v[m][j]);
+    }
+  }
+  __syncthreads();
+}
+
+// Last stage (q * RR == R, so k == b) writing its outputs straight to HBM.
+template <typename T, int RR, int E>
+__device__ __noinline__ void stage_last(LdsC<T> *buf, int R, int q, int twstep, const GlbC<T> *__restrict__ tw,
+                                        FLAGFFT_DST_PARAMS, int lt, int tpu) {
+  constexpr int MAXB = (E + RR - 1) / RR;
+  const int nb = R / RR;
+  V2<T> v[MAXB][RR];
+#pragma unroll
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = lt + m * tpu;
+    if (b < nb) {
+#pragma unroll
+      for (int j = 0; j < RR; ++j) v[m][j] = ldc(buf + lpad(b + j * nb));
+      const int st = b * twstep;
+#pragma unroll
+      for (int j = 1; j < RR; ++j) v[m][j] = cmul(v[m][j], ldg(tw + j * st));
+    }
+  }
+  if (!d_active) return;
+#pragma unroll
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = lt + m * tpu;
+    if (b < nb) {
+      dft<RR>(v[m]);
+#pragma unroll
+      for (int j = 0; j < RR; ++j)
+        dst_elem<T>(d_out, d_base, d_es, d_mode, d_nlog, d_chirp, d_bhat, d_conj, d_scale, b + j * q, v[m][j]);
+    }
+  }
+}
+
+// Single-stage transform (R <= 16): HBM -> registers -> HBM.
+template <typename T, int RR, int E>
+__device__ __noinline__ void stage_only(FLAGFFT_SRC_PARAMS, FLAGFFT_DST_PARAMS, int lt) {
+  if (lt != 0 || !s_active) return;
+  V2<T> v[RR];
+#pragma unroll
+  for (int j = 0; j < RR; ++j) v[j] = src_elem<T>(s_in, s_base, s_es, s_mode, s_nlog, s_chirp, s_conj, j);
+  dft<RR>(v);
+#pragma unroll
+  for (int j = 0; j < RR; ++j) dst_elem<T>(d_out, d_base, d_es, d_mode, d_nlog, d_chirp, d_bhat, d_conj, d_scale, j, v[j]);
+}
+
+#define FLAGFFT_RADICES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
 // Forward DFT of length R held in LDS (natural order in, natural order out).
 // Every thread of the block must call this (it contains block barriers).
 template <typename T, int E>
-__device__ __forceinline__ void lds_fft(V2<T> *buf, int R, int nst, const int32_t *rad, const FastDiv *fdq, const int32_t *twstep,
-                        int lt, int tpu, const V2<T> *__restrict__ tw) {
+__device__ __forceinline__ void lds_fft(LdsC<T> *buf, int R, int nst, const int32_t *rad, const FastDiv *fdq,
+                                        const int32_t *twstep, int lt, int tpu, const void *tw_small) {
+  const GlbC<T> *tw = gc<T>(tw_small);
   int q = 1;
   for (int s = 0; s < nst; ++s) {
     const int r = rad[s];
     switch (r) {
-      case 2: lds_stage<T, 2, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 3: lds_stage<T, 3, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 4: lds_stage<T, 4, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 5: lds_stage<T, 5, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 6: lds_stage<T, 6, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 7: lds_stage<T, 7, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 8: lds_stage<T, 8, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 9: lds_stage<T, 9, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 10: lds_stage<T, 10, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 11: lds_stage<T, 11, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 12: lds_stage<T, 12, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 13: lds_stage<T, 13, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 14: lds_stage<T, 14, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 15: lds_stage<T, 15, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      case 16: lds_stage<T, 16, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
-      default: break;  // the planner never emits other radices
+#define FLAGFFT_MID(RR) \
+  case RR: lds_stage<T, RR, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
+      FLAGFFT_RADICES(FLAGFFT_MID)
+#undef FLAGFFT_MID
+      default: break;  // the planner emits radices 2..16 only
     }
     q *= r;
   }
 }
 
+// Same transform with the first stage fed from HBM and the last stage writing HBM.
+template <typename T, int E>
+__device__ __forceinline__ void lds_fft_direct(LdsC<T> *buf, int R, int nst, const int32_t *rad, const FastDiv *fdq,
+                                               const int32_t *twstep, int lt, int tpu, const void *tw_small,
+                                               const SrcDesc &src, const DstDesc<T> &dst) {
+  const GlbC<T> *tw = gc<T>(tw_small);
+  if (nst == 0) {  // length 1
+    if (lt == 0 && src.active)
+      dst_elem<T>(dst.out, dst.base, dst.es, dst.mode, dst.nlog, dst.chirp, dst.bhat, dst.conj, dst.scale, 0,
+                  src_elem<T>(src.in, src.base, src.es, src.mode, src.nlog, src.chirp, src.conj, 0));
+    return;
+  }
+  if (nst == 1) {
+    switch (rad[0]) {
+#define FLAGFFT_ONLY(RR) \
+  case RR: stage_only<T, RR, E>(FLAGFFT_SRC_ARGS(src), FLAGFFT_DST_ARGS(dst), lt); break;
+      FLAGFFT_RADICES(FLAGFFT_ONLY)
+#undef FLAGFFT_ONLY
+      default: break;
+    }
+    return;
+  }
+  switch (rad[0]) {
+#define FLAGFFT_FIRST(RR) \
+  case RR: stage_first<T, RR, E>(buf, R, FLAGFFT_SRC_ARGS(src), lt, tpu); break;
+    FLAGFFT_RADICES(FLAGFFT_FIRST)
+#undef FLAGFFT_FIRST
+    default: break;
+  }
+  int q = rad[0];
+  for (int s = 1; s + 1 < nst; ++s) {
+    const int r = rad[s];
+    switch (r) {
+#define FLAGFFT_MID(RR) \
+  case RR: lds_stage<T, RR, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
+      FLAGFFT_RADICES(FLAGFFT_MID)
+#undef FLAGFFT_MID
+      default: break;
+    }
+    q *= r;
+  }
+  switch (rad[nst - 1]) {
+#define FLAGFFT_LAST(RR) \
+  case RR: stage_last<T, RR, E>(buf, R, q, twstep[nst - 1], tw, FLAGFFT_DST_ARGS(dst), lt, tpu); break;
+    FLAGFFT_RADICES(FLAGFFT_LAST)
+#undef FLAGFFT_LAST
+    default: break;
+  }
+}
+
+// Unit u -> (b, c, i) of a pass.
+__device__ __forceinline__ void decompose_unit(const PassArgs &a, uint32_t uu, uint32_t &b, uint32_t &c, uint32_t &i) {
+  if (a.col_fast) {
+    const uint32_t q1 = fdiv(uu, a.fd_ncol);
+    c = uu - q1 * a.ncol;
+    b = fdiv(q1, a.fd_tcnt);
+    i = q1 - b * a.tcnt;
+  } else {
+    const uint32_t q1 = fdiv(uu, a.fd_tcnt);
+    i = uu - q1 * a.tcnt;
+    b = fdiv(q1, a.fd_ncol);
+    c = q1 - b * a.ncol;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Generic pass: staged load (+source mode, conj, super-stage twiddle) ->
-// LDS transform -> staged store (+sink mode).
+// Generic pass.  Direct form (contiguous single-pass sequences): first stage
+// reads HBM, last stage writes HBM.  Staged form (strided sequences and
+// multi-pass super-stages): batched coalesced copy into LDS with the source
+// mode, conjugation and super-stage twiddle applied, LDS transform, batched
+// coalesced copy out with the sink mode.
 // ---------------------------------------------------------------------------
 template <typename T, int E>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V2<T> *lds = reinterpret_cast<V2<T> *>(smem_raw);
+  LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int blk = remap_block(blockIdx.x, a.nblocks, a.xcd_remap);
   const int64_t u0 = static_cast<int64_t>(blk) * a.C;
-  const int CR = a.C * a.R;
   const int LS = a.lds_stride;
-  const V2<T> *tw_lo = static_cast<const V2<T> *>(a.tw_lo);
-  const V2<T> *tw_hi = static_cast<const V2<T> *>(a.tw_hi);
-  const V2<T> *chirp = static_cast<const V2<T> *>(a.chirp);
-  const uint32_t tw_mask = (1u << a.tw_bits) - 1u;
+  const T scale = static_cast<T>(a.scale);
 
-  for (int f = tid; f < CR; f += nthr) {
-    int s, e;
-    if (a.load_slot_fast) {
+  if (a.direct) {
+    const int s = tid / a.tpu;
+    const int lt = tid - s * a.tpu;
+    const int64_t u = u0 + s;
+    const bool active = u < a.units;
+    uint32_t b = 0, c = 0, i = 0;
+    if (active) decompose_unit(a, static_cast<uint32_t>(u), b, c, i);
+    SrcDesc src{a.in, static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs, a.gi.es, a.nlog,
+                a.chirp, a.src_mode, a.conj_in, active ? 1 : 0};
+    DstDesc<T> dst{a.out, static_cast<int64_t>(b) * a.go.bs + static_cast<int64_t>(c) * a.go.cs, a.go.es, a.nlog,
+                   a.chirp, a.bhat, a.dst_mode, a.conj_out, active ? 1 : 0, scale};
+    lds_fft_direct<T, E>(lds + s * LS, a.R, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small, src, dst);
+    return;
+  }
+
+  const int CR = a.C * a.R;
+  const GlbC<T> *tw_lo = gc<T>(a.tw_lo);
+  const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
+  const uint32_t tw_mask = (1u << a.tw_bits) - 1u;
+  auto split = [&](int f, int slot_fast, int &s, int &e) __attribute__((always_inline)) {
+    if (slot_fast) {
       e = static_cast<int>(fdiv(f, a.fd_C));
       s = f - e * a.C;
     } else {
       s = static_cast<int>(fdiv(f, a.fd_R));
       e = f - s * a.R;
     }
-    const int64_t u = u0 + s;
-    V2<T> v{0, 0};
-    if (u < a.units) {
-      const uint32_t uu = static_cast<uint32_t>(u);
-      uint32_t i, c, b;
-      if (a.col_fast) {
-        const uint32_t q1 = fdiv(uu, a.fd_ncol);
-        c = uu - q1 * a.ncol;
-        b = fdiv(q1, a.fd_tcnt);
-        i = q1 - b * a.tcnt;
-      } else {
-        const uint32_t q1 = fdiv(uu, a.fd_tcnt);
-        i = uu - q1 * a.tcnt;
-        b = fdiv(q1, a.fd_ncol);
-        c = q1 - b * a.ncol;
-      }
-      const int64_t base = static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs;
-      const int64_t elog = static_cast<int64_t>(i) + static_cast<int64_t>(e) * a.tcnt;
-      v = src_elem<T>(a.in, base, a.gi.es, a.src_mode, a.nlog, chirp, a.conj_in, elog);
-      if (a.p > 1 && e > 0) {
-        const uint32_t k = i - fdiv(i, a.fd_p) * a.p;
-        const uint32_t mm = static_cast<uint32_t>(e) * k * static_cast<uint32_t>(a.twp_mult);
-        v = cmul(v, cmul(tw_hi[mm >> a.tw_bits], tw_lo[mm & tw_mask]));
-      }
-    }
-    lds[s * LS + lpad(e)] = v;
-  }
+  };
+  batched<T, kBatch>(
+      CR, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int s, e;
+        split(f, a.load_slot_fast, s, e);
+        const int64_t u = u0 + s;
+        V2<T> v{0, 0};
+        if (u < a.units) {
+          uint32_t b, c, i;
+          decompose_unit(a, static_cast<uint32_t>(u), b, c, i);
+          const int64_t base = static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs;
+          const int64_t elog = static_cast<int64_t>(i) + static_cast<int64_t>(e) * a.tcnt;
+          v = src_elem<T>(a.in, base, a.gi.es, a.src_mode, a.nlog, a.chirp, a.conj_in, elog);
+          if (a.p > 1 && e > 0) {
+            const uint32_t k = i - fdiv(i, a.fd_p) * a.p;
+            const uint32_t mm = static_cast<uint32_t>(e) * k * static_cast<uint32_t>(a.twp_mult);
+            v = cmul(v, cmul(ldg(tw_hi + (mm >> a.tw_bits)), ldg(tw_lo + (mm & tw_mask))));
+          }
+        }
+        return v;
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int s, e;
+        split(f, a.load_slot_fast, s, e);
+        stc(lds + s * LS + lpad(e), v);
+      });
   __syncthreads();
 
   {
     const int s = tid / a.tpu;
     const int lt = tid - s * a.tpu;
-    lds_fft<T, E>(lds + s * LS, a.R, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu,
-                  static_cast<const V2<T> *>(a.tw_small));
+    lds_fft<T, E>(lds + s * LS, a.R, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   }
 
-  const V2<T> *bhat = static_cast<const V2<T> *>(a.bhat);
-  const T scale = static_cast<T>(a.scale);
-  for (int f = tid; f < CR; f += nthr) {
-    int s, e;
-    if (a.store_slot_fast) {
-      e = static_cast<int>(fdiv(f, a.fd_C));
-      s = f - e * a.C;
-    } else {
-      s = static_cast<int>(fdiv(f, a.fd_R));
-      e = f - s * a.R;
-    }
-    const int64_t u = u0 + s;
-    if (u >= a.units) continue;
-    const uint32_t uu = static_cast<uint32_t>(u);
-    uint32_t i, c, b;
-    if (a.col_fast) {
-      const uint32_t q1 = fdiv(uu, a.fd_ncol);
-      c = uu - q1 * a.ncol;
-      b = fdiv(q1, a.fd_tcnt);
-      i = q1 - b * a.tcnt;
-    } else {
-      const uint32_t q1 = fdiv(uu, a.fd_tcnt);
-      i = uu - q1 * a.tcnt;
-      b = fdiv(q1, a.fd_ncol);
-      c = q1 - b * a.ncol;
-    }
-    const uint32_t k = (a.p > 1) ? i - fdiv(i, a.fd_p) * a.p : 0u;
-    const int64_t oidx = static_cast<int64_t>(i - k) * a.R + k + static_cast<int64_t>(e) * a.p;
-    const int64_t base = static_cast<int64_t>(b) * a.go.bs + static_cast<int64_t>(c) * a.go.cs;
-    dst_elem<T>(a.out, base, a.go.es, a.dst_mode, a.nlog, chirp, bhat, a.conj_out, scale, oidx,
-                lds[s * LS + lpad(e)]);
-  }
+  batched<T, kBatch>(
+      CR, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int s, e;
+        split(f, a.store_slot_fast, s, e);
+        return ldc(lds + s * LS + lpad(e));
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int s, e;
+        split(f, a.store_slot_fast, s, e);
+        const int64_t u = u0 + s;
+        if (u >= a.units) return;
+        uint32_t b, c, i;
+        decompose_unit(a, static_cast<uint32_t>(u), b, c, i);
+        const uint32_t k = (a.p > 1) ? i - fdiv(i, a.fd_p) * a.p : 0u;
+        const int64_t oidx = static_cast<int64_t>(i - k) * a.R + k + static_cast<int64_t>(e) * a.p;
+        const int64_t base = static_cast<int64_t>(b) * a.go.bs + static_cast<int64_t>(c) * a.go.cs;
+        dst_elem<T>(a.out, base, a.go.es, a.dst_mode, a.nlog, a.chirp, a.bhat, a.conj_out, scale, oidx, v);
+      });
 }
 
 // Sequence q -> base offset of (c, b) for the given geometry.
@@ -284,58 +392,68 @@ __device__ __forceinline__ void slot_elem(int f, int slot_fast, const FusedArgs 
 // ---------------------------------------------------------------------------
 template <typename T, int E>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V2<T> *lds = reinterpret_cast<V2<T> *>(smem_raw);
+  LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
   const int p = a.n, m = a.m, LS = a.lds_stride;
   const int xoff = 0, aoff = lpad(p) + 1, dcoff = aoff + lpad(m) + 1;
-  const int32_t *idx = static_cast<const int32_t *>(a.aux);
-  const V2<T> *bhat = static_cast<const V2<T> *>(a.bhat);
+  const GlbR<int32_t> *idx = gr<int32_t>(a.aux);
+  const GlbC<T> *bhat = gc<T>(a.bhat);
 
-  for (int f = tid; f < a.C * p; f += nthr) {
-    int s, e;
-    slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, s, e);
-    const int64_t q = q0 + s;
-    V2<T> v{0, 0};
-    if (q < a.nseqs) {
-      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode, p, nullptr, a.conj_in,
-                      e);
-    }
-    lds[s * LS + xoff + lpad(e)] = v;
-  }
+  batched<T, kBatch>(
+      a.C * p, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int s, e;
+        slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, s, e);
+        const int64_t q = q0 + s;
+        if (q >= a.nseqs) return V2<T>{0, 0};
+        return src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode, p, nullptr,
+                           a.conj_in, e);
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int s, e;
+        slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, s, e);
+        stc(lds + s * LS + xoff + lpad(e), v);
+      });
   __syncthreads();
   const int s = tid / a.tpu;
   const int lt = tid - s * a.tpu;
-  V2<T> *X = lds + s * LS + xoff;
-  V2<T> *A = lds + s * LS + aoff;
+  LdsC<T> *X = lds + s * LS + xoff;
+  LdsC<T> *A = lds + s * LS + aoff;
   for (int qq = lt; qq < m; qq += a.tpu) {
     const int src = idx[qq == 0 ? 0 : m - qq];
-    A[lpad(qq)] = X[lpad(src)];
+    stc(A + lpad(qq), ldc(X + lpad(src)));
   }
   __syncthreads();
-  lds_fft<T, E>(A, m, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, static_cast<const V2<T> *>(a.tw_small));
+  lds_fft<T, E>(A, m, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   for (int qq = lt; qq < m; qq += a.tpu) {
-    const V2<T> av = A[lpad(qq)];
-    if (qq == 0) lds[s * LS + dcoff] = av;  // sum over x_j, j != 0
-    A[lpad(qq)] = cconj(cmul(av, bhat[qq]));
+    const V2<T> av = ldc(A + lpad(qq));
+    if (qq == 0) stc(lds + s * LS + dcoff, av);  // sum over x_j, j != 0
+    stc(A + lpad(qq), cconj(cmul(av, ldg(bhat + qq))));
   }
   __syncthreads();
-  lds_fft<T, E>(A, m, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, static_cast<const V2<T> *>(a.tw_small));
-  const V2<T> x0 = X[0];
-  for (int qq = lt; qq < m; qq += a.tpu) X[lpad(idx[qq])] = cadd(x0, cconj(A[lpad(qq)]));
+  lds_fft<T, E>(A, m, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
+  const V2<T> x0 = ldc(X);
+  for (int qq = lt; qq < m; qq += a.tpu) stc(X + lpad(idx[qq]), cadd(x0, cconj(ldc(A + lpad(qq)))));
   __syncthreads();
-  if (lt == 0) X[0] = cadd(x0, lds[s * LS + dcoff]);
+  if (lt == 0) stc(X, cadd(x0, ldc(lds + s * LS + dcoff)));
   __syncthreads();
   const T scale = static_cast<T>(a.scale);
-  for (int f = tid; f < a.C * a.sext; f += nthr) {
-    int ss, e;
-    slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
-    const int64_t q = q0 + ss;
-    if (q >= a.nseqs) continue;
-    dst_elem<T>(a.out, seq_base(static_cast<uint32_t>(q), a, a.go), a.go.es, a.dst_mode, p, nullptr, nullptr,
-                a.conj_out, scale, e, lds[ss * LS + xoff + lpad(e)]);
-  }
+  batched<T, kBatch>(
+      a.C * a.sext, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
+        return ldc(lds + ss * LS + xoff + lpad(e));
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
+        const int64_t q = q0 + ss;
+        if (q >= a.nseqs) return;
+        dst_elem<T>(a.out, seq_base(static_cast<uint32_t>(q), a, a.go), a.go.es, a.dst_mode, p, nullptr, nullptr,
+                    a.conj_out, scale, e, v);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -345,45 +463,53 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
 // ---------------------------------------------------------------------------
 template <typename T, int E>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V2<T> *lds = reinterpret_cast<V2<T> *>(smem_raw);
+  LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
   const int n = a.n, M = a.m, LS = a.lds_stride;
-  const V2<T> *chirp = static_cast<const V2<T> *>(a.chirp);
-  const V2<T> *bhat = static_cast<const V2<T> *>(a.bhat);
+  const GlbC<T> *chirp = gc<T>(a.chirp);
+  const GlbC<T> *bhat = gc<T>(a.bhat);
   const int s = tid / a.tpu;
   const int lt = tid - s * a.tpu;
-  V2<T> *A = lds + s * LS;
-  for (int e = n + lt; e < M; e += a.tpu) A[lpad(e)] = V2<T>{0, 0};
-  for (int f = tid; f < a.C * n; f += nthr) {
-    int ss, e;
-    slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, ss, e);
-    const int64_t q = q0 + ss;
-    V2<T> v{0, 0};
-    if (q < a.nseqs) {
-      v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode & 3, n, nullptr,
-                      a.conj_in, e);
-      v = cmul(v, chirp[e]);
-    }
-    lds[ss * LS + lpad(e)] = v;
-  }
+  LdsC<T> *A = lds + s * LS;
+  for (int e = n + lt; e < M; e += a.tpu) stc(A + lpad(e), V2<T>{0, 0});
+  batched<T, kBatch>(
+      a.C * n, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, ss, e);
+        const int64_t q = q0 + ss;
+        if (q >= a.nseqs) return V2<T>{0, 0};
+        const V2<T> v = src_elem<T>(a.in, seq_base(static_cast<uint32_t>(q), a, a.gi), a.gi.es, a.src_mode & 3, n,
+                                    nullptr, a.conj_in, e);
+        return cmul(v, ldg(chirp + e));
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.load_slot_fast, a, a.lext, a.fd_lext, ss, e);
+        stc(lds + ss * LS + lpad(e), v);
+      });
   __syncthreads();
-  const V2<T> *tw = static_cast<const V2<T> *>(a.tw_small);
-  lds_fft<T, E>(A, M, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, tw);
-  for (int qq = lt; qq < M; qq += a.tpu) A[lpad(qq)] = cconj(cmul(A[lpad(qq)], bhat[qq]));
+  lds_fft<T, E>(A, M, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
+  for (int qq = lt; qq < M; qq += a.tpu) stc(A + lpad(qq), cconj(cmul(ldc(A + lpad(qq)), ldg(bhat + qq))));
   __syncthreads();
-  lds_fft<T, E>(A, M, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, tw);
+  lds_fft<T, E>(A, M, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   const T scale = static_cast<T>(a.scale);
-  for (int f = tid; f < a.C * a.sext; f += nthr) {
-    int ss, e;
-    slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
-    const int64_t q = q0 + ss;
-    if (q >= a.nseqs) continue;
-    V2<T> v = cmul(cconj(lds[ss * LS + lpad(e)]), chirp[e]);
-    dst_elem<T>(a.out, seq_base(static_cast<uint32_t>(q), a, a.go), a.go.es, a.dst_mode & 3, n, nullptr, nullptr,
-                a.conj_out, scale, e, v);
-  }
+  batched<T, kBatch>(
+      a.C * a.sext, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
+        return cmul(cconj(ldc(lds + ss * LS + lpad(e))), ldg(chirp + e));
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int ss, e;
+        slot_elem(f, a.store_slot_fast, a, a.sext, a.fd_sext, ss, e);
+        const int64_t q = q0 + ss;
+        if (q >= a.nseqs) return;
+        dst_elem<T>(a.out, seq_base(static_cast<uint32_t>(q), a, a.go), a.go.es, a.dst_mode & 3, n, nullptr,
+                    nullptr, a.conj_out, scale, e, v);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -394,44 +520,53 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs 
 // ---------------------------------------------------------------------------
 template <typename T, int E>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V2<T> *lds = reinterpret_cast<V2<T> *>(smem_raw);
+  LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
   const int h = a.m, LS = a.lds_stride;
-  const V2<T> *wn = static_cast<const V2<T> *>(a.aux);
-  for (int f = tid; f < a.C * h; f += nthr) {
-    int s, e;
-    slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, e);
-    const int64_t q = q0 + s;
-    V2<T> v{0, 0};
-    if (q < a.nseqs) {
-      const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.gi);
-      v = *reinterpret_cast<const V2<T> *>(static_cast<const T *>(a.in) + base + 2 * static_cast<int64_t>(e));
-    }
-    lds[s * LS + lpad(e)] = v;
-  }
+  const GlbC<T> *wn = gc<T>(a.aux);
+  batched<T, kBatch>(
+      a.C * h, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int s, e;
+        slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, e);
+        const int64_t q = q0 + s;
+        if (q >= a.nseqs) return V2<T>{0, 0};
+        const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.gi);
+        return ldg((const GlbC<T> *)(gr<T>(a.in) + base + 2 * static_cast<int64_t>(e)));
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int s, e;
+        slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, e);
+        stc(lds + s * LS + lpad(e), v);
+      });
   __syncthreads();
   const int s = tid / a.tpu;
   const int lt = tid - s * a.tpu;
-  lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu,
-                static_cast<const V2<T> *>(a.tw_small));
+  lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   const T half = T(0.5);
-  for (int f = tid; f < a.C * (h + 1); f += nthr) {
-    int ss, k;
-    slot_elem(f, a.store_slot_fast, a, h + 1, a.fd_sext, ss, k);
-    const int64_t q = q0 + ss;
-    if (q >= a.nseqs) continue;
-    const V2<T> *Z = lds + ss * LS;
-    const V2<T> zk = Z[lpad(k == h ? 0 : k)];
-    const V2<T> zc = cconj(Z[lpad(k == 0 ? 0 : h - k)]);
-    const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
-    const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
-    V2<T> X = cadd(ev, cmul(wn[k], od));
-    if (a.conj_out) X.y = -X.y;
-    const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.go);
-    static_cast<V2<T> *>(a.out)[base + static_cast<int64_t>(k) * a.go.es] = X;
-  }
+  batched<T, kBatch>(
+      a.C * (h + 1), tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int ss, k;
+        slot_elem(f, a.store_slot_fast, a, h + 1, a.fd_sext, ss, k);
+        const LdsC<T> *Z = lds + ss * LS;
+        const V2<T> zk = ldc(Z + lpad(k == h ? 0 : k));
+        const V2<T> zc = cconj(ldc(Z + lpad(k == 0 ? 0 : h - k)));
+        const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
+        const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
+        V2<T> X = cadd(ev, cmul(ldg(wn + k), od));
+        if (a.conj_out) X.y = -X.y;
+        return X;
+      },
+      [&](int f, V2<T> X) __attribute__((always_inline)) {
+        int ss, k;
+        slot_elem(f, a.store_slot_fast, a, h + 1, a.fd_sext, ss, k);
+        const int64_t q = q0 + ss;
+        if (q >= a.nseqs) return;
+        const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.go);
+        stg(gc<T>(a.out) + base + static_cast<int64_t>(k) * a.go.es, X);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -442,52 +577,61 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a
 // ---------------------------------------------------------------------------
 template <typename T, int E>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V2<T> *lds = reinterpret_cast<V2<T> *>(smem_raw);
+  LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
   const int h = a.m, LS = a.lds_stride;
-  const V2<T> *wn = static_cast<const V2<T> *>(a.aux);
-  const V2<T> *X = static_cast<const V2<T> *>(a.in);
-  for (int f = tid; f < a.C * h; f += nthr) {
-    int s, k;
-    slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, k);
-    const int64_t q = q0 + s;
-    V2<T> v{0, 0};
-    if (q < a.nseqs) {
-      const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.gi);
-      V2<T> xk = X[base + static_cast<int64_t>(k) * a.gi.es];
-      V2<T> xh = X[base + static_cast<int64_t>(h - k) * a.gi.es];
-      if (k == 0) {
-        xk.y = T(0);
-        xh.y = T(0);
-      }
-      xh.y = -xh.y;  // conj X_{h-k}
-      const V2<T> sum = cadd(xk, xh);
-      const V2<T> dif = csub(xk, xh);
-      // i * conj(w_n^k) * dif
-      const V2<T> t = cmul_pi(cmul(dif, cconj(wn[k])));
-      v = cconj(cadd(sum, t));  // conj for the inverse-by-forward trick
-    }
-    lds[s * LS + lpad(k)] = v;
-  }
+  const GlbC<T> *wn = gc<T>(a.aux);
+  const GlbC<T> *X = gc<T>(a.in);
+  batched<T, kBatch>(
+      a.C * h, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int s, k;
+        slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, k);
+        const int64_t q = q0 + s;
+        if (q >= a.nseqs) return V2<T>{0, 0};
+        const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.gi);
+        V2<T> xk = ldg(X + base + static_cast<int64_t>(k) * a.gi.es);
+        V2<T> xh = ldg(X + base + static_cast<int64_t>(h - k) * a.gi.es);
+        if (k == 0) {
+          xk.y = T(0);
+          xh.y = T(0);
+        }
+        xh.y = -xh.y;  // conj X_{h-k}
+        const V2<T> sum = cadd(xk, xh);
+        const V2<T> dif = csub(xk, xh);
+        // i * conj(w_n^k) * dif
+        const V2<T> t = cmul_pi(cmul(dif, cconj(ldg(wn + k))));
+        return cconj(cadd(sum, t));  // conj for the inverse-by-forward identity
+      },
+      [&](int f, V2<T> v) __attribute__((always_inline)) {
+        int s, k;
+        slot_elem(f, a.load_slot_fast, a, h, a.fd_lext, s, k);
+        stc(lds + s * LS + lpad(k), v);
+      });
   __syncthreads();
   const int s = tid / a.tpu;
   const int lt = tid - s * a.tpu;
-  lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu,
-                static_cast<const V2<T> *>(a.tw_small));
+  lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   const T scale = static_cast<T>(a.scale);
-  for (int f = tid; f < a.C * h; f += nthr) {
-    int ss, j;
-    slot_elem(f, a.store_slot_fast, a, h, a.fd_sext, ss, j);
-    const int64_t q = q0 + ss;
-    if (q >= a.nseqs) continue;
-    V2<T> z = cconj(lds[ss * LS + lpad(j)]);
-    z.x *= scale;
-    z.y *= scale;
-    const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.go);
-    *reinterpret_cast<V2<T> *>(static_cast<T *>(a.out) + base + 2 * static_cast<int64_t>(j)) = z;
-  }
+  batched<T, kBatch>(
+      a.C * h, tid, nthr,
+      [&](int f) __attribute__((always_inline)) {
+        int ss, j;
+        slot_elem(f, a.store_slot_fast, a, h, a.fd_sext, ss, j);
+        V2<T> z = cconj(ldc(lds + ss * LS + lpad(j)));
+        z.x *= scale;
+        z.y *= scale;
+        return z;
+      },
+      [&](int f, V2<T> z) __attribute__((always_inline)) {
+        int ss, j;
+        slot_elem(f, a.store_slot_fast, a, h, a.fd_sext, ss, j);
+        const int64_t q = q0 + ss;
+        if (q >= a.nseqs) return;
+        const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.go);
+        stg((GlbC<T> *)(gr<T>(a.out) + base + 2 * static_cast<int64_t>(j)), z);
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -498,21 +642,19 @@ template <typename T>
 __global__ void k_real_post(const RealPassArgs a) {
   const int h = a.n / 2;
   const int64_t total = a.nseqs * (h + 1);
-  const V2<T> *Zall = static_cast<const V2<T> *>(a.in);
-  const V2<T> *wn = static_cast<const V2<T> *>(a.tw_half);
+  const GlbC<T> *wn = gc<T>(a.tw_half);
   const T half = T(0.5);
   for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
        g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t q = g / (h + 1);
     const int k = static_cast<int>(g - q * (h + 1));
     const int64_t b = q / a.ncol, c = q - b * a.ncol;
-    const V2<T> *Z = Zall + b * a.gi.bs + c * a.gi.cs;
-    const V2<T> zk = Z[(k == h ? 0 : k) * a.gi.es];
-    const V2<T> zc = cconj(Z[(k == 0 ? 0 : h - k) * a.gi.es]);
+    const GlbC<T> *Z = gc<T>(a.in) + b * a.gi.bs + c * a.gi.cs;
+    const V2<T> zk = ldg(Z + (k == h ? 0 : k) * a.gi.es);
+    const V2<T> zc = cconj(ldg(Z + (k == 0 ? 0 : h - k) * a.gi.es));
     const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
     const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
-    V2<T> X = cadd(ev, cmul(wn[k], od));
-    static_cast<V2<T> *>(a.out)[b * a.go.bs + c * a.go.cs + k * a.go.es] = X;
+    stg(gc<T>(a.out) + b * a.go.bs + c * a.go.cs + k * a.go.es, cadd(ev, cmul(ldg(wn + k), od)));
   }
 }
 
@@ -520,22 +662,22 @@ template <typename T>
 __global__ void k_real_pre(const RealPassArgs a) {
   const int h = a.n / 2;
   const int64_t total = a.nseqs * h;
-  const V2<T> *wn = static_cast<const V2<T> *>(a.tw_half);
+  const GlbC<T> *wn = gc<T>(a.tw_half);
   for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
        g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t q = g / h;
     const int k = static_cast<int>(g - q * h);
     const int64_t b = q / a.ncol, c = q - b * a.ncol;
-    const V2<T> *Xs = static_cast<const V2<T> *>(a.in) + b * a.gi.bs + c * a.gi.cs;
-    V2<T> xk = Xs[k * a.gi.es];
-    V2<T> xh = Xs[(h - k) * a.gi.es];
+    const GlbC<T> *Xs = gc<T>(a.in) + b * a.gi.bs + c * a.gi.cs;
+    V2<T> xk = ldg(Xs + k * a.gi.es);
+    V2<T> xh = ldg(Xs + (h - k) * a.gi.es);
     if (k == 0) {
       xk.y = T(0);
       xh.y = T(0);
     }
     xh.y = -xh.y;
-    const V2<T> t = cmul_pi(cmul(csub(xk, xh), cconj(wn[k])));
-    static_cast<V2<T> *>(a.out)[b * a.go.bs + c * a.go.cs + k * a.go.es] = cconj(cadd(cadd(xk, xh), t));
+    const V2<T> t = cmul_pi(cmul(csub(xk, xh), cconj(ldg(wn + k))));
+    stg(gc<T>(a.out) + b * a.go.bs + c * a.go.cs + k * a.go.es, cconj(cadd(cadd(xk, xh), t)));
   }
 }
 
@@ -564,34 +706,34 @@ __global__ void k_gen_chirp(V2<T> *out, int64_t n) {
   }
 }
 
-__global__ void k_gen_bluestein_b(double2 *out, int64_t n, int64_t m) {
+__global__ void k_gen_bluestein_b(V2<double> *out, int64_t n, int64_t m) {
   for (int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; j < m;
        j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     int64_t k = -1;
     if (j < n) k = j;
     else if (m - j < n) k = m - j;
     if (k < 0) {
-      out[j] = double2{0.0, 0.0};
+      out[j] = V2<double>{0.0, 0.0};
       continue;
     }
     const uint64_t r = (static_cast<uint64_t>(k) * static_cast<uint64_t>(k)) % static_cast<uint64_t>(2 * n);
     double s, c;
     sincospi(static_cast<double>(r) / static_cast<double>(n), &s, &c);
-    out[j] = double2{c, s};
+    out[j] = V2<double>{c, s};
   }
 }
 
-__global__ void k_gen_rader_b(double2 *out, const int32_t *idx, int64_t p) {
+__global__ void k_gen_rader_b(V2<double> *out, const int32_t *idx, int64_t p) {
   for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < p - 1;
        q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     double s, c;
     sincospi(-2.0 * static_cast<double>(idx[q]) / static_cast<double>(p), &s, &c);
-    out[q] = double2{c, s};
+    out[q] = V2<double>{c, s};
   }
 }
 
 template <typename T>
-__global__ void k_scale_cast(const double2 *in, V2<T> *out, int64_t count, double scale) {
+__global__ void k_scale_cast(const V2<double> *in, V2<T> *out, int64_t count, double scale) {
   for (int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; j < count;
        j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     out[j] = V2<T>{static_cast<T>(in[j].x * scale), static_cast<T>(in[j].y * scale)};
@@ -631,7 +773,7 @@ dim3 grid_for(int64_t total, int threads) {
 
 }  // namespace
 
-bool supported_ept(bool dbl, int ept) { return dbl ? (ept == 8 || ept == 16) : (ept == 16 || ept == 32); }
+bool supported_ept(bool dbl, int ept) { return dbl ? (ept == 8 || ept == 16) : (ept == 8 || ept == 16 || ept == 32); }
 
 #define FLAGFFT_DISPATCH_EPT(KERNEL, ARGS)                                                  \
   do {                                                                                      \
@@ -639,6 +781,7 @@ bool supported_ept(bool dbl, int ept) { return dbl ? (ept == 8 || ept == 16) : (
       if (ept == 8) return launch_one(dev::KERNEL<double, 8>, ARGS, s, st);                 \
       if (ept == 16) return launch_one(dev::KERNEL<double, 16>, ARGS, s, st);               \
     } else {                                                                                \
+      if (ept == 8) return launch_one(dev::KERNEL<float, 8>, ARGS, s, st);                  \
       if (ept == 16) return launch_one(dev::KERNEL<float, 16>, ARGS, s, st);                \
       if (ept == 32) return launch_one(dev::KERNEL<float, 32>, ARGS, s, st);                \
     }                                                                                       \
@@ -677,32 +820,32 @@ hipError_t launch_real_pre(const RealPassArgs &a, bool dbl, hipStream_t st) {
 
 hipError_t gen_roots(void *out, bool dbl, int64_t count, int64_t n, int64_t mult, hipStream_t st) {
   dim3 g = grid_for(count, 256);
-  if (dbl) hipLaunchKernelGGL(dev::k_gen_roots<double>, g, dim3(256), 0, st, static_cast<double2 *>(out), count, n, mult);
-  else hipLaunchKernelGGL(dev::k_gen_roots<float>, g, dim3(256), 0, st, static_cast<float2 *>(out), count, n, mult);
+  if (dbl) hipLaunchKernelGGL(dev::k_gen_roots<double>, g, dim3(256), 0, st, static_cast<dev::V2<double> *>(out), count, n, mult);
+  else hipLaunchKernelGGL(dev::k_gen_roots<float>, g, dim3(256), 0, st, static_cast<dev::V2<float> *>(out), count, n, mult);
   return hipGetLastError();
 }
 
 hipError_t gen_chirp(void *out, bool dbl, int64_t n, hipStream_t st) {
   dim3 g = grid_for(n, 256);
-  if (dbl) hipLaunchKernelGGL(dev::k_gen_chirp<double>, g, dim3(256), 0, st, static_cast<double2 *>(out), n);
-  else hipLaunchKernelGGL(dev::k_gen_chirp<float>, g, dim3(256), 0, st, static_cast<float2 *>(out), n);
+  if (dbl) hipLaunchKernelGGL(dev::k_gen_chirp<double>, g, dim3(256), 0, st, static_cast<dev::V2<double> *>(out), n);
+  else hipLaunchKernelGGL(dev::k_gen_chirp<float>, g, dim3(256), 0, st, static_cast<dev::V2<float> *>(out), n);
   return hipGetLastError();
 }
 
 hipError_t gen_bluestein_b(double2 *out, int64_t n, int64_t m, hipStream_t st) {
-  hipLaunchKernelGGL(dev::k_gen_bluestein_b, grid_for(m, 256), dim3(256), 0, st, out, n, m);
+  hipLaunchKernelGGL(dev::k_gen_bluestein_b, grid_for(m, 256), dim3(256), 0, st, reinterpret_cast<dev::V2<double> *>(out), n, m);
   return hipGetLastError();
 }
 
 hipError_t gen_rader_b(double2 *out, const int32_t *idx, int64_t p, hipStream_t st) {
-  hipLaunchKernelGGL(dev::k_gen_rader_b, grid_for(p - 1, 256), dim3(256), 0, st, out, idx, p);
+  hipLaunchKernelGGL(dev::k_gen_rader_b, grid_for(p - 1, 256), dim3(256), 0, st, reinterpret_cast<dev::V2<double> *>(out), idx, p);
   return hipGetLastError();
 }
 
 hipError_t scale_cast(const double2 *in, void *out, bool dbl, int64_t count, double scale, hipStream_t st) {
   dim3 g = grid_for(count, 256);
-  if (dbl) hipLaunchKernelGGL(dev::k_scale_cast<double>, g, dim3(256), 0, st, in, static_cast<double2 *>(out), count, scale);
-  else hipLaunchKernelGGL(dev::k_scale_cast<float>, g, dim3(256), 0, st, in, static_cast<float2 *>(out), count, scale);
+  if (dbl) hipLaunchKernelGGL(dev::k_scale_cast<double>, g, dim3(256), 0, st, reinterpret_cast<const dev::V2<double> *>(in), static_cast<dev::V2<double> *>(out), count, scale);
+  else hipLaunchKernelGGL(dev::k_scale_cast<float>, g, dim3(256), 0, st, reinterpret_cast<const dev::V2<double> *>(in), static_cast<dev::V2<float> *>(out), count, scale);
   return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "fft_args.hpp"
 
 namespace flagfft {
@@ -14,7 +16,7 @@ struct LaunchShape {
 };
 
 // Elements-per-thread variants compiled for every kernel family.
-bool supported_ept(bool dbl, int ept);  // fp32: 16, 32; fp64: 8, 16
+bool supported_ept(bool dbl, int ept);  // fp32: 8, 16, 32; fp64: 8, 16
 constexpr int kMaxBlockThreads = 512;
 
 hipError_t launch_pass(const PassArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
@@ -22,6 +24,11 @@ hipError_t launch_rader(const FusedArgs &a, bool dbl, int ept, const LaunchShape
 hipError_t launch_bluestein(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
 hipError_t launch_r2c_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
 hipError_t launch_c2r_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
+// Specialised single-pass kernels: id of the compiled radix sequence (-1 if none).
+int leaf_kernel_id(const std::vector<int> &radices);
+int leaf_kernel_ept(int id);
+hipError_t launch_leaf(const LeafArgs &a, bool dbl, int id, const LaunchShape &s, hipStream_t st);
+
 hipError_t launch_real_post(const RealPassArgs &a, bool dbl, hipStream_t st);
 hipError_t launch_real_pre(const RealPassArgs &a, bool dbl, hipStream_t st);
 

@@ -135,6 +135,27 @@ def fft_f32(x, sign: int = -1, nthreads: int = 0) -> np.ndarray:
     return out
 
 
+def rfft_f32(x, nthreads: int = 0) -> np.ndarray:
+    """Fast FP32 R2C (the FP32 CPU baseline path)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[-1]
+    h = n // 2 + 1
+    out = np.empty(x.shape[:-1] + (h,), dtype=np.complex64)
+    rc = lib().ora_fft_r2c_f(_ptr(x, _fp), _ptr(out.view(np.float32), _fp), n, x.size // n, n, h, nthreads)
+    assert rc == 0
+    return out
+
+
+def irfft_f32(X, n: int, nthreads: int = 0) -> np.ndarray:
+    """Fast FP32 unnormalised C2R."""
+    X = np.ascontiguousarray(X, dtype=np.complex64)
+    h = n // 2 + 1
+    out = np.empty(X.shape[:-1] + (n,), dtype=np.float32)
+    rc = lib().ora_fft_c2r_f(_ptr(X.view(np.float32), _fp), _ptr(out, _fp), n, X.size // h, h, n, nthreads)
+    assert rc == 0
+    return out
+
+
 def rfft(x, nthreads: int = 0) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float64)
     n = x.shape[-1]
