@@ -134,13 +134,22 @@ SUITE = [  # the other configurations, reported with --suite (not the headline l
 
 
 def kernel_symbol(label: str) -> str:
-    """Step label 'k_pass<fp32,16> ...' -> the kernel symbol rocprof reports."""
+    """Step label -> the kernel symbol rocprof reports.
+
+    'k_pass<fp32,16> ...'            -> 'k_pass<float, 16>'
+    'k_leaf<fp32,16>[16x16x16] ...'   -> 'k_leaf<float, 16, 16, 16, 16>'
+    'k_stage<fp32,16,8>[16x10] ...'   -> 'k_stage<float, 16, 8, 16, 10>'
+    """
     head = label.split(" ")[0]
-    fam, _, targs = head.partition("<")
-    if not targs:
+    fam, _, rest = head.partition("<")
+    if not rest:
         return fam
-    prec, ept = targs.rstrip(">").split(",")
-    return f"{fam}<{'float' if prec == 'fp32' else 'double'}, {ept}>"
+    targs, _, rad = rest.partition(">")
+    parts = targs.split(",")
+    parts[0] = "float" if parts[0] == "fp32" else "double"
+    if rad.startswith("["):
+        parts += rad.strip("[]").split("x")
+    return f"{fam}<{', '.join(parts)}>"
 
 
 def measure(cases, steps, warmup, torch, ff, stream):
@@ -201,7 +210,7 @@ def roofline_of(launches):
         for k, v in by_sym.items()}
 
 
-def cpu_baseline(cases, budget_s=4.0):
+def cpu_baseline(cases, budget_s=12.0):
     from oracle import oracle as ora
 
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -217,7 +226,7 @@ def cpu_baseline(cases, budget_s=4.0):
             total_t += dt
             total_b += b
         reps += 1
-        if reps >= 20:
+        if reps >= 200:
             break
     return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
             "sample": f"{reps} x the same step with each batch cut to {int(frac * 100)}% "

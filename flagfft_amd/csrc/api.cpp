@@ -35,6 +35,7 @@ struct Desc {
   flagfftType type = FLAGFFT_C2C;
   Prec prec = Prec::F32;
   bool real_fwd = false, real_inv = false;
+  int compute = 0;
 };
 
 struct PlanImpl {
@@ -43,6 +44,7 @@ struct PlanImpl {
   PlanTree tree;
   std::unique_ptr<Executable> fwd, inv;
   hipStream_t stream = nullptr;
+  bool graph_mode = false;  // Exec replays one hipGraph per direction
   std::string description;
   std::mutex mu;
 };
@@ -125,6 +127,7 @@ PlanDesc plan_desc(const Desc &d) {
   p.prec = d.prec;
   p.real_fwd = d.real_fwd;
   p.real_inv = d.real_inv;
+  p.compute = d.compute;
   return p;
 }
 
@@ -192,14 +195,14 @@ flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagf
   if (p->desc.type != expect) return FLAGFFT_INVALID_TYPE;
   if (has_direction && direction != FLAGFFT_FORWARD && direction != FLAGFFT_INVERSE) return FLAGFFT_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(p->mu);
-  const Executable *ex = nullptr;
+  Executable *ex = nullptr;
   if (p->desc.real_fwd) ex = p->fwd.get();
   else if (p->desc.real_inv) ex = p->inv.get();
   else ex = direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
   if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
   int cur = p->device;
   if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);
-  hipError_t e = ex->run(in, out, p->stream);
+  hipError_t e = (p->graph_mode && !ex->profiling) ? ex->run_graph(in, out, p->stream) : ex->run(in, out, p->stream);
   if (cur != p->device) (void)hipSetDevice(cur);
   if (e != hipSuccess) {
     std::fprintf(stderr, "[flagfft] exec failed: %s\n", hipGetErrorString(e));
@@ -229,6 +232,21 @@ extern "C" flagfftResult flagfftPlan1d(flagfftHandle *plan, int nx, flagfftType 
   const bool real_inv = type == FLAGFFT_C2R || type == FLAGFFT_Z2D;
   const int half = nx / 2 + 1;
   return flagfftPlanMany(plan, 1, n, nullptr, 1, real_inv ? half : nx, nullptr, 1, real_fwd ? half : nx, type, batch);
+}
+
+extern "C" flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType type, int batch,
+                                              int compute) {
+  if (plan == nullptr) return FLAGFFT_INVALID_VALUE;
+  *plan = nullptr;
+  if (compute < FLAGFFTX_COMPUTE_VECTOR || compute > FLAGFFTX_COMPUTE_FP16X3) return FLAGFFT_INVALID_VALUE;
+  if (compute == FLAGFFTX_COMPUTE_VECTOR) return flagfftPlan1d(plan, nx, type, batch);
+  int n[1] = {nx};
+  flagfft::Desc d;
+  flagfftResult r = flagfft::make_desc(1, n, nullptr, 1, nx, nullptr, 1, nx, type, batch, d);
+  if (r != FLAGFFT_SUCCESS) return r;
+  if (type != FLAGFFT_C2C || nx != 1024) return FLAGFFT_NOT_SUPPORTED;
+  d.compute = compute;
+  return flagfft::build_plan(plan, d);
 }
 
 extern "C" flagfftResult flagfftPlan2d(flagfftHandle *plan, int nx, int ny, flagfftType type) {
@@ -400,6 +418,14 @@ extern "C" const char *flagfftxStepLabel(flagfftHandle handle, int direction, in
   flagfft::Executable *ex = flagfft::exec_for(p, direction);
   if (ex == nullptr || step < 0 || step >= static_cast<int>(ex->steps.size())) return nullptr;
   return ex->steps[static_cast<size_t>(step)].label.c_str();
+}
+
+extern "C" flagfftResult flagfftxSetGraphMode(flagfftHandle handle, int enable) {
+  flagfft::PlanImpl *p = flagfft::checked(handle);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->graph_mode = enable != 0;
+  return FLAGFFT_SUCCESS;
 }
 
 extern "C" flagfftResult flagfftxSetTuning(const flagfftxTuning *t) {

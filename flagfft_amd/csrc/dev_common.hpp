@@ -128,5 +128,25 @@ __device__ __forceinline__ void dst_elem(void *out, int64_t base, int64_t es, in
   }
 }
 
+// Batched staged copy: every thread issues U independent loads before it
+// commits any of them, so U global (or LDS) round trips overlap instead of
+// serialising one per loop iteration.
+template <typename T, int U, typename Load, typename Commit>
+__device__ __forceinline__ void batched(int total, int tid, int nthr, Load load, Commit commit) {
+  for (int f0 = tid; f0 < total; f0 += U * nthr) {
+    V2<T> v[U];
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int f = f0 + uu * nthr;
+      if (f < total) v[uu] = load(f);
+    }
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int f = f0 + uu * nthr;
+      if (f < total) commit(f, v[uu]);
+    }
+  }
+}
+
 }  // namespace dev
 }  // namespace flagfft

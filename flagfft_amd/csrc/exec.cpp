@@ -8,6 +8,12 @@
 
 namespace flagfft {
 
+namespace {
+// Workgroups of the matrix-core kernel: each of its 4 waves loops over
+// transforms; 1024 groups keep every CU at its register-limited occupancy.
+constexpr int64_t kMfmaGroups = 1024;
+}  // namespace
+
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw HipError(e, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -33,6 +39,8 @@ void Executable::allocate_workspaces() {
 }
 
 Executable::~Executable() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
   for (auto &p : ev_pending_) {
     (void)hipEventDestroy(p.second.start);
     (void)hipEventDestroy(p.second.stop);
@@ -83,16 +91,129 @@ hipError_t Executable::harvest_profile() const {
   return first_err;
 }
 
-hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
-  auto resolve = [&](int id) -> void * {
+size_t Step::pack_args(void *src, void *dst, unsigned char *buf) const {
+  auto put = [&](auto a) {
+    static_assert(sizeof(a) <= kArgBytes, "kernel argument struct too large");
+    a.in = src;
+    a.out = dst;
+    std::memcpy(buf, &a, sizeof(a));
+    return sizeof(a);
+  };
+  switch (kind) {
+    case Pass: return put(pa);
+    case Leaf: return put(la);
+    case Stage: return put(sa);
+    case Mfma: return put(ma);
+    case Rader:
+    case Bluestein:
+    case R2CEven:
+    case C2REven: return put(fa);
+    case RealPost:
+    case RealPre: return put(ra);
+  }
+  return 0;
+}
+
+Step Step::rebatched(int64_t batch, int64_t built) const {
+  Step s = *this;
+  if (batch == built) return s;
+  auto scale = [&](int64_t v) { return v / built * batch; };
+  auto blocks = [](int64_t units, int64_t per) { return std::max<int64_t>(1, (units + per - 1) / per); };
+  switch (kind) {
+    case Pass:
+      s.pa.units = scale(pa.units);
+      s.pa.nblocks = static_cast<int32_t>(blocks(s.pa.units, pa.C));
+      s.shape.grid = dim3(static_cast<unsigned>(s.pa.nblocks));
+      break;
+    case Stage: {
+      int per = 1, ept = 0;  // units per workgroup of the specialised kernel
+      stage_kernel_shape(leaf_id, dbl, per, ept);
+      s.sa.units = scale(sa.units);
+      s.sa.nblocks = static_cast<int32_t>(blocks(s.sa.units, per));
+      s.shape.grid = dim3(static_cast<unsigned>(s.sa.nblocks));
+      break;
+    }
+    case Leaf:
+      s.la.nseq = scale(la.nseq);
+      s.shape.grid = dim3(static_cast<unsigned>(blocks(s.la.nseq, la.C)));
+      break;
+    case Mfma:
+      s.ma.nseq = scale(ma.nseq);
+      s.shape.grid = dim3(static_cast<unsigned>(std::min<int64_t>(blocks(s.ma.nseq, 4), kMfmaGroups)));
+      break;
+    case Rader:
+    case Bluestein:
+    case R2CEven:
+    case C2REven:
+      s.fa.nseqs = scale(fa.nseqs);
+      s.shape.grid = dim3(static_cast<unsigned>(blocks(s.fa.nseqs, fa.C)));
+      break;
+    case RealPost:
+      s.ra.nseqs = scale(ra.nseqs);
+      s.shape = real_post_shape(s.ra);
+      break;
+    case RealPre:
+      s.ra.nseqs = scale(ra.nseqs);
+      s.shape = real_pre_shape(s.ra);
+      break;
+  }
+  return s;
+}
+
+hipError_t Executable::run_batch(const void *in, void *out, void *const *wsp, int64_t batch, hipStream_t st) const {
+  alignas(16) unsigned char buf[Step::kArgBytes];
+  auto res = [&](int id) -> void * {
     if (id == kBufIn) return const_cast<void *>(in);
     if (id == kBufOut) return out;
-    return ws[static_cast<size_t>(id - kBufWs0)]->ptr;
+    return wsp[id - kBufWs0];
   };
+  for (const Step &s0 : steps) {
+    const Step s = s0.rebatched(batch, built_batch);
+    s.pack_args(res(s.src), res(s.dst), buf);
+    void *kp[] = {buf};
+    hipError_t e = hipLaunchKernel(s.func, s.shape.grid, s.shape.block, kp, s.shape.lds_bytes, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void Executable::resolve_kernels() {
+  for (Step &s : steps) {
+    switch (s.kind) {
+      case Step::Pass: s.func = pass_kernel(s.dbl, s.ept); break;
+      case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id); break;
+      case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id); break;
+      case Step::Mfma: s.func = mfma_kernel(s.compute); break;
+      case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
+      case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
+      case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
+      case Step::C2REven: s.func = c2r_even_kernel(s.dbl, s.ept); break;
+      case Step::RealPost:
+        s.func = real_post_kernel(s.dbl);
+        s.shape = real_post_shape(s.ra);
+        break;
+      case Step::RealPre:
+        s.func = real_pre_kernel(s.dbl);
+        s.shape = real_pre_shape(s.ra);
+        break;
+    }
+    if (s.func == nullptr) throw std::runtime_error("no compiled kernel for step: " + s.label);
+    hip_check(allow_lds(s.func, s.shape.lds_bytes), "hipFuncSetAttribute");
+  }
+}
+
+void *Executable::buffer(int id, const void *in, void *out) const {
+  if (id == kBufIn) return const_cast<void *>(in);
+  if (id == kBufOut) return out;
+  return ws[static_cast<size_t>(id - kBufWs0)]->ptr;
+}
+
+hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
+  alignas(16) unsigned char buf[Step::kArgBytes];
   for (size_t si = 0; si < steps.size(); ++si) {
     const Step &s = steps[si];
-    void *src = resolve(s.src);
-    void *dst = resolve(s.dst);
+    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf);
+    void *kp[] = {buf};
     hipError_t e = hipSuccess;
     EvPair ev;
     if (profiling) {
@@ -100,43 +221,7 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
       if (e == hipSuccess) e = hipEventRecord(ev.start, st);
       if (e != hipSuccess) return e;
     }
-    switch (s.kind) {
-      case Step::Pass: {
-        PassArgs a = s.pa;
-        a.in = src;
-        a.out = dst;
-        e = launch_pass(a, s.dbl, s.ept, s.shape, st);
-        break;
-      }
-      case Step::Leaf: {
-        LeafArgs a = s.la;
-        a.in = src;
-        a.out = dst;
-        e = launch_leaf(a, s.dbl, s.leaf_id, s.shape, st);
-        break;
-      }
-      case Step::Rader:
-      case Step::Bluestein:
-      case Step::R2CEven:
-      case Step::C2REven: {
-        FusedArgs a = s.fa;
-        a.in = src;
-        a.out = dst;
-        if (s.kind == Step::Rader) e = launch_rader(a, s.dbl, s.ept, s.shape, st);
-        else if (s.kind == Step::Bluestein) e = launch_bluestein(a, s.dbl, s.ept, s.shape, st);
-        else if (s.kind == Step::R2CEven) e = launch_r2c_even(a, s.dbl, s.ept, s.shape, st);
-        else e = launch_c2r_even(a, s.dbl, s.ept, s.shape, st);
-        break;
-      }
-      case Step::RealPost:
-      case Step::RealPre: {
-        RealPassArgs a = s.ra;
-        a.in = src;
-        a.out = dst;
-        e = s.kind == Step::RealPost ? launch_real_post(a, s.dbl, st) : launch_real_pre(a, s.dbl, st);
-        break;
-      }
-    }
+    e = hipLaunchKernel(s.func, s.shape.grid, s.shape.block, kp, s.shape.lds_bytes, st);
     if (e != hipSuccess) return e;
     if (profiling) {
       e = hipEventRecord(ev.stop, st);
@@ -145,6 +230,54 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
     }
   }
   return hipSuccess;
+}
+
+hipError_t Executable::run_graph(const void *in, void *out, hipStream_t st) {
+  alignas(16) unsigned char buf[Step::kArgBytes];
+  auto params = [&](const Step &s, void **kp) {
+    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf);
+    kp[0] = buf;
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(s.func);
+    p.gridDim = s.shape.grid;
+    p.blockDim = s.shape.block;
+    p.sharedMemBytes = static_cast<unsigned>(s.shape.lds_bytes);
+    p.kernelParams = kp;
+    p.extra = nullptr;
+    return p;
+  };
+  hipError_t e = hipSuccess;
+  if (graph_exec_ == nullptr) {
+    e = hipGraphCreate(&graph_, 0);
+    if (e != hipSuccess) return e;
+    graph_nodes_.clear();
+    for (const Step &s : steps) {
+      void *kp[1];
+      hipKernelNodeParams p = params(s, kp);
+      hipGraphNode_t node;
+      const hipGraphNode_t *dep = graph_nodes_.empty() ? nullptr : &graph_nodes_.back();
+      e = hipGraphAddKernelNode(&node, graph_, dep, dep ? 1 : 0, &p);
+      if (e != hipSuccess) return e;
+      graph_nodes_.push_back(node);
+    }
+    e = hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      graph_exec_ = nullptr;
+      return e;
+    }
+  } else if (in != graph_in_ || out != graph_out_) {
+    for (size_t i = 0; i < steps.size(); ++i) {
+      const Step &s = steps[i];
+      if (s.src != kBufIn && s.src != kBufOut && s.dst != kBufIn && s.dst != kBufOut) continue;
+      void *kp[1];
+      hipKernelNodeParams p = params(s, kp);
+      e = hipGraphExecKernelNodeSetParams(graph_exec_, graph_nodes_[i], &p);
+      if (e != hipSuccess) return e;
+    }
+  }
+  graph_in_ = in;
+  graph_out_ = out;
+  return hipGraphLaunch(graph_exec_, st);
 }
 
 namespace {
@@ -261,6 +394,7 @@ const void *Builder::fp64_transform_then_cast(const std::string &key, const std:
     b64.emit_core(c64, s, kBufIn, kBufOut, m);
     b64.finish();
   }
+  ex64->resolve_kernels();
   ex64->allocate_workspaces();
   auto out64 = std::make_shared<DevBuf>(static_cast<size_t>(len) * 16);
   hip_check(ex64->run(src64->ptr, out64->ptr, st_), "FP64 convolution kernel transform");
@@ -350,7 +484,7 @@ ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, b
     if (o.ept && ept != o.ept) continue;
     int tpu = static_cast<int>((work_len + ept - 1) / ept);
     if (tpu < 1) tpu = 1;
-    if (tpu > kMaxBlockThreads) continue;
+    if (tpu > o.max_threads) continue;
     int desired = chunked ? std::max(1, o.chunk_bytes / eb) : std::max(1, o.target_threads / tpu);
     if (!chunked && (nunits + desired - 1) / desired < kTargetGroups)
       desired = static_cast<int>(std::max<int64_t>(1, nunits / kTargetGroups));
@@ -358,7 +492,7 @@ ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, b
     if (desired > nunits) desired = static_cast<int>(std::max<int64_t>(nunits, 1));
     const size_t cap = static_cast<size_t>(chunked ? o.lds_cap_bytes : o.lds_pref_bytes);
     int C = desired;
-    while (C > 1 && (C * tpu > kMaxBlockThreads || static_cast<size_t>(C) * LS * eb > cap)) --C;
+    while (C > 1 && (C * tpu > o.max_threads || static_cast<size_t>(C) * LS * eb > cap)) --C;
     if (static_cast<size_t>(C) * LS * eb > static_cast<size_t>(o.lds_cap_bytes)) continue;
     ShapeChoice sc;
     sc.C = C;
@@ -386,7 +520,7 @@ ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, b
 
 void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
   // Compile-time specialised kernel for contiguous complex rows of a listed radix sequence.
-  const int leaf_id = leaf_kernel_id(c.radices);
+  const int leaf_id = opt_.generic_kernels ? -1 : leaf_kernel_id(c.radices);
   if (leaf_id >= 0 && !s.col_fast && s.ncol == 1 && s.gi.es == 1 && s.go.es == 1 && m.src_mode == kSrcComplex &&
       m.dst_mode == kDstComplex && c.n > 1) {
     Step st;
@@ -398,7 +532,7 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
     LeafArgs &a = st.la;
     const int R = static_cast<int>(c.n);
     const int ept = leaf_kernel_ept(leaf_id);
-    const int tpu = R / ept;
+    const int tpu = (R + ept - 1) / ept;
     const int LS = odd_stride(lds_pad(R - 1) + 1);
     a.nseq = s.nbatch;
     a.idist = s.gi.bs;
@@ -552,6 +686,58 @@ void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, c
     a.bhat = nullptr;  // the pointwise pass of a Bluestein chain gets its table from emit_bluestein
     if ((a.src_mode & kSrcChirpFlag) || (a.dst_mode & kDstChirpFlag)) a.chirp = chirp(a.nlog);
     const int LS = odd_stride(lds_pad(R - 1) + 1);
+    // compile-time specialised super-stage for contiguous complex sequences
+    const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(c.pass_radices[j]);
+    if (sid >= 0 && !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1 && (a.src_mode & 3) == kSrcComplex &&
+        (a.dst_mode & 3) == kDstComplex) {
+      int cu = 0, ept = 0;
+      stage_kernel_shape(sid, dbl(), cu, ept);
+      const int tpu = (R + ept - 1) / ept;
+      Step sst;
+      sst.kind = Step::Stage;
+      sst.dbl = dbl();
+      sst.src = st.src;
+      sst.dst = st.dst;
+      sst.leaf_id = sid;
+      StageArgs &g = sst.sa;
+      g.bs_in = a.gi.bs;
+      g.bs_out = a.go.bs;
+      g.tcnt = static_cast<int32_t>(tcnt);
+      g.p = static_cast<int32_t>(p);
+      g.twp_mult = a.twp_mult;
+      g.tw_bits = bits;
+      g.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
+      g.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+      g.tw_small = a.tw_small;
+      g.tw_lo = tw_lo;
+      g.tw_hi = tw_hi;
+      g.conj_in = a.conj_in;
+      g.conj_out = a.conj_out;
+      g.scale = a.scale;
+      g.units = a.units;
+      g.block_store = (p == 1 && a.dst_mode == kDstComplex && a.go.bs == n) ? 1 : 0;
+      g.src_mode = a.src_mode;
+      g.dst_mode = a.dst_mode;
+      g.nlog = a.nlog;
+      g.chirp = a.chirp;
+      g.bhat = nullptr;
+      const int64_t nblocks = (a.units + cu - 1) / cu;
+      g.nblocks = static_cast<int32_t>(nblocks);
+      g.xcd_remap = opt_.xcd_remap;
+      sst.ept = ept;
+      sst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cu * tpu)),
+                              static_cast<size_t>(cu) * LS * eb()};
+      sst.bytes_moved = s.nbatch * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+      std::ostringstream lab;
+      lab << "k_stage<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << cu << ">[";
+      for (size_t ri = 0; ri < c.pass_radices[j].size(); ++ri) lab << (ri ? "x" : "") << c.pass_radices[j][ri];
+      lab << "] pass " << j << "/" << k << " R=" << R << " p=" << p << " tcnt=" << tcnt
+          << (g.block_store ? " block-store" : "");
+      sst.label = lab.str();
+      ex_.steps.push_back(sst);
+      p *= R;
+      continue;
+    }
     const int64_t in_i_stride = a.gi.es;
     const int64_t out_i_stride = (p > 1) ? a.go.es : static_cast<int64_t>(R) * a.go.es;
     const bool lsf = slot_fast(tcnt * a.gi.es, unit_stride(s.col_fast, tcnt, s.ncol, a.gi, in_i_stride));
@@ -637,6 +823,36 @@ void Builder::emit_rader(const Core &c, const SeqSet &s, int src, int dst, const
   ex_.steps.push_back(st);
 }
 
+void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  if (c.n != 1024 || dbl() || s.col_fast || s.ncol != 1 || s.gi.es != 1 || s.go.es != 1 || m.src_mode != kSrcComplex ||
+      m.dst_mode != kDstComplex)
+    throw std::invalid_argument("matrix-core transform needs contiguous complex FP32 rows of length 1024");
+  Step st;
+  st.kind = Step::Mfma;
+  st.src = src;
+  st.dst = dst;
+  st.compute = c.compute;
+  MfmaArgs &a = st.ma;
+  a.nseq = s.nbatch;
+  a.idist = s.gi.bs;
+  a.odist = s.go.bs;
+  a.tw = roots(1024, 1024, 1);
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  // 4 waves per workgroup, each wave loops over transforms; 1024 workgroups
+  // keep every CU at its register-limited occupancy (2 waves per SIMD)
+  const int64_t waves = (s.nbatch + 3) / 4;
+  const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(waves, opt_.forced_units > 0 ? opt_.forced_units : kMfmaGroups));
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(groups)), dim3(256), 0};
+  st.bytes_moved = s.nbatch * 1024 * 8 * 2;
+  std::ostringstream lab;
+  lab << "k_mfma1024<" << (c.compute == kComputeFp16x3 ? "fp16x3" : "bf16x3") << "> R=1024 transforms=" << s.nbatch
+      << " groups=" << groups;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+}
+
 void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
   const int64_t n = c.n, M = c.m;
   if (c.conv) {
@@ -654,7 +870,12 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     mid.dst_mode = kDstPointwiseFlag;
     mid.nlog = n;
     emit_multipass(*c.conv, sa, src, wP, first, mid, {dst});
-    ex_.steps.back().pa.bhat = bhat_bluestein(n, M);
+    const void *bh = bhat_bluestein(n, M);
+    Step &pw = ex_.steps.back();
+    if (pw.kind == Step::Stage)
+      pw.sa.bhat = bh;
+    else
+      pw.pa.bhat = bh;
     Modes mfirst;
     mfirst.nlog = n;
     Modes fin = m;
@@ -720,6 +941,7 @@ void Builder::emit_core(const Core &c, const SeqSet &s, int src, int dst, const 
     case CoreKind::MultiPass: emit_multipass(c, s, src, dst, m, m, {}); break;
     case CoreKind::Rader: emit_rader(c, s, src, dst, m); break;
     case CoreKind::Bluestein: emit_bluestein(c, s, src, dst, m); break;
+    case CoreKind::MatrixCore: emit_matrix_core(c, s, src, dst, m); break;
   }
 }
 
@@ -905,6 +1127,8 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
     }
   }
   b.finish();
+  ex->built_batch = t.desc.rank == 1 ? t.desc.batch : 1;
+  ex->resolve_kernels();
   ex->allocate_workspaces();
   return ex;
 }

@@ -39,18 +39,29 @@ struct DevBuf {
 enum BufId : int { kBufIn = 0, kBufOut = 1, kBufWs0 = 2 };
 
 struct Step {
-  enum Kind { Pass, Leaf, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre } kind = Pass;
+  enum Kind { Pass, Leaf, Stage, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre, Mfma } kind = Pass;
   bool dbl = false;
   int ept = 16;
-  int leaf_id = -1;  // Leaf: entry of the specialised kernel table
+  int leaf_id = -1;  // Leaf / Stage: entry of the specialised kernel table
   int src = kBufIn, dst = kBufOut;
   PassArgs pa{};
   LeafArgs la{};
+  StageArgs sa{};
   FusedArgs fa{};
   RealPassArgs ra{};
+  MfmaArgs ma{};
+  int compute = 0;  // Mfma: kComputeBf16x3 / kComputeFp16x3
   LaunchShape shape{};
   int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)
   std::string label;
+  const void *func = nullptr;  // kernel handle, set by Executable::resolve_kernels()
+
+  static constexpr size_t kArgBytes = 1024;
+  // The step's argument struct with its in/out pointers set; returns its size.
+  size_t pack_args(void *src, void *dst, unsigned char *buf) const;
+  // The same launch for `batch` sequences of a 1D plan built for `built` (every
+  // per-batch count of a 1D plan is proportional to the batch).
+  Step rebatched(int64_t batch, int64_t built) const;
 };
 
 class Executable {
@@ -67,6 +78,15 @@ class Executable {
 
   // Enqueue every step on `st`; in/out are device pointers.
   hipError_t run(const void *in, void *out, hipStream_t st) const;
+  // Same launches as one hipGraph: recorded on first use, later calls patch
+  // only the kernel nodes that read `in` or write `out`, then launch it.
+  hipError_t run_graph(const void *in, void *out, hipStream_t st);
+  // Look up every step's kernel handle and raise LDS limits (after building).
+  void resolve_kernels();
+  // 1D plans: run for `batch` sequences with caller-provided workspaces
+  // (ws[i] of ws_bytes[i] / built_batch * batch bytes).
+  hipError_t run_batch(const void *in, void *out, void *const *ws, int64_t batch, hipStream_t st) const;
+  int64_t built_batch = 1;
   std::string describe() const;
   void allocate_workspaces();
 
@@ -84,6 +104,12 @@ class Executable {
   mutable std::vector<EvPair> ev_free_;
   mutable std::vector<std::pair<size_t, EvPair>> ev_pending_;
   hipError_t take_pair(EvPair &p) const;
+  void *buffer(int id, const void *in, void *out) const;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<hipGraphNode_t> graph_nodes_;
+  const void *graph_in_ = nullptr;
+  void *graph_out_ = nullptr;
 };
 
 // Geometry of a batch of equal-length sequences in one buffer.
@@ -133,6 +159,7 @@ class Builder {
                       std::initializer_list<int> avoid_extra);
   void emit_rader(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
                    int64_t R);
 };

@@ -129,6 +129,59 @@ struct LeafArgs {
   int32_t C;             // transforms per workgroup
 };
 
+// Compile-time specialised Stockham super-stage over contiguous complex
+// sequences (unit u -> (b = u / tcnt, i = u % tcnt)).
+struct StageArgs {
+  const void *in;
+  void *out;
+  int64_t bs_in, bs_out;  // complex elements between sequences
+  int32_t tcnt, p, twp_mult, tw_bits;
+  FastDiv fd_tcnt, fd_p;
+  const void *tw_small;   // w_R^j
+  const void *tw_lo;      // w_nseq^l, l < 2^tw_bits
+  const void *tw_hi;      // w_nseq^(h 2^tw_bits)
+  int32_t conj_in, conj_out;
+  double scale;
+  int32_t nblocks, xcd_remap;
+  int64_t units;          // nbatch * tcnt
+  int32_t block_store;    // the C units' outputs are one contiguous block (first pass, contiguous rows)
+  int32_t src_mode, dst_mode;  // complex, or the Bluestein chirp source / pointwise and chirp sinks
+  int64_t nlog;           // user length for the chirp modes
+  const void *chirp, *bhat;
+};
+
+// Split-precision matrix-core transform of length 1024 (contiguous complex rows).
+enum ComputeMode : int32_t { kComputeVector = 0, kComputeBf16x3 = 1, kComputeFp16x3 = 2 };
+struct MfmaArgs {
+  const void *in;
+  void *out;
+  int64_t nseq;          // transforms
+  int64_t idist, odist;  // complex elements between consecutive transforms
+  const void *tw;        // FP32 w_1024^j, j < 1024
+  int32_t conj_in, conj_out;
+  double scale;
+};
+
+// One job of the persistent mixed-length kernel: a single-pass transform of
+// `nblocks` workgroup tiles whose arguments are those of the equivalent
+// standalone launch (FP32 kernels with 16 elements per thread, FP64 with 8).
+enum QueueKind : int32_t { kQueuePass = 0, kQueueRader = 1, kQueueBluestein = 2, kQueueR2CEven = 3, kQueueC2REven = 4 };
+struct QueueJob {
+  int32_t kind;     // QueueKind
+  int32_t dbl;
+  int32_t first;    // index of the job's first tile in the launch
+  int32_t nblocks;  // tiles
+  PassArgs pa;      // kQueuePass
+  FusedArgs fa;     // the other kinds
+};
+struct QueueArgs {
+  const QueueJob *jobs;  // sorted by `first`
+  int32_t njobs;
+  int32_t total;         // tiles over all jobs
+  uint32_t *counter;     // next tile to hand out; 0 at launch
+};
+constexpr int kQueueThreads = 256;
+
 // Split-spectrum pre/post pass for even-length real transforms that do not fit one LDS transform.
 struct RealPassArgs {
   const void *in;

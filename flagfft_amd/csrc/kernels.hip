@@ -23,26 +23,6 @@
 namespace flagfft {
 namespace dev {
 
-// Batched staged copy: every thread issues U independent loads before it
-// commits any of them, so U global (or LDS) round trips overlap instead of
-// serialising one per loop iteration.
-template <typename T, int U, typename Load, typename Commit>
-__device__ __forceinline__ void batched(int total, int tid, int nthr, Load load, Commit commit) {
-  for (int f0 = tid; f0 < total; f0 += U * nthr) {
-    V2<T> v[U];
-#pragma unroll
-    for (int uu = 0; uu < U; ++uu) {
-      const int f = f0 + uu * nthr;
-      if (f < total) v[uu] = load(f);
-    }
-#pragma unroll
-    for (int uu = 0; uu < U; ++uu) {
-      const int f = f0 + uu * nthr;
-      if (f < total) commit(f, v[uu]);
-    }
-  }
-}
-
 constexpr int kBatch = 8;
 
 // Where the first stage reads and the last stage writes when they talk to HBM directly.
@@ -273,21 +253,32 @@ __device__ __forceinline__ void decompose_unit(const PassArgs &a, uint32_t uu, u
 // mode, conjugation and super-stage twiddle applied, LDS transform, batched
 // coalesced copy out with the sink mode.
 // ---------------------------------------------------------------------------
+// Slot s and lane lt of thread tid for C slots of tpu threads; threads past
+// C * tpu (a persistent workgroup running a smaller tile) get no butterflies
+// but still reach every barrier.
+__device__ __forceinline__ void slot_lane(int tid, int C, int tpu, int &s, int &lt) {
+  s = tid / tpu;
+  lt = tid - s * tpu;
+  if (s >= C) {
+    s = 0;
+    lt = 1 << 24;
+  }
+}
+
 template <typename T, int E>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
+__device__ __forceinline__ void pass_body(const PassArgs &a, int blk) {
   LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
-  const int blk = remap_block(blockIdx.x, a.nblocks, a.xcd_remap);
   const int64_t u0 = static_cast<int64_t>(blk) * a.C;
   const int LS = a.lds_stride;
   const T scale = static_cast<T>(a.scale);
 
   if (a.direct) {
-    const int s = tid / a.tpu;
-    const int lt = tid - s * a.tpu;
+    int s, lt;
+    slot_lane(tid, a.C, a.tpu, s, lt);
     const int64_t u = u0 + s;
-    const bool active = u < a.units;
+    const bool active = u < a.units && lt < a.tpu;
     uint32_t b = 0, c = 0, i = 0;
     if (active) decompose_unit(a, static_cast<uint32_t>(u), b, c, i);
     SrcDesc src{a.in, static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs, a.gi.es, a.nlog,
@@ -340,8 +331,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
   __syncthreads();
 
   {
-    const int s = tid / a.tpu;
-    const int lt = tid - s * a.tpu;
+    int s, lt;
+    slot_lane(tid, a.C, a.tpu, s, lt);
     lds_fft<T, E>(lds + s * LS, a.R, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   }
 
@@ -364,6 +355,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
         const int64_t base = static_cast<int64_t>(b) * a.go.bs + static_cast<int64_t>(c) * a.go.cs;
         dst_elem<T>(a.out, base, a.go.es, a.dst_mode, a.nlog, a.chirp, a.bhat, a.conj_out, scale, oidx, v);
       });
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_pass(const PassArgs a) {
+  pass_body<T, E>(a, remap_block(blockIdx.x, a.nblocks, a.xcd_remap));
 }
 
 // Sequence q -> base offset of (c, b) for the given geometry.
@@ -391,10 +387,10 @@ __device__ __forceinline__ void slot_elem(int f, int slot_fast, const FusedArgs 
 // LDS per slot: [0, p) x / X, [p, p + m) a / A / W, one cell for A[0].
 // ---------------------------------------------------------------------------
 template <typename T, int E>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
+__device__ __forceinline__ void rader_body(const FusedArgs &a, int blk) {
   LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
+  const int64_t q0 = static_cast<int64_t>(blk) * a.C;
   const int p = a.n, m = a.m, LS = a.lds_stride;
   const int xoff = 0, aoff = lpad(p) + 1, dcoff = aoff + lpad(m) + 1;
   const GlbR<int32_t> *idx = gr<int32_t>(a.aux);
@@ -416,8 +412,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
         stc(lds + s * LS + xoff + lpad(e), v);
       });
   __syncthreads();
-  const int s = tid / a.tpu;
-  const int lt = tid - s * a.tpu;
+  int s, lt;
+  slot_lane(tid, a.C, a.tpu, s, lt);
   LdsC<T> *X = lds + s * LS + xoff;
   LdsC<T> *A = lds + s * LS + aoff;
   for (int qq = lt; qq < m; qq += a.tpu) {
@@ -456,21 +452,26 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
       });
 }
 
+template <typename T, int E>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_rader(const FusedArgs a) {
+  rader_body<T, E>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Bluestein, whole convolution in LDS (src/exec/raw_nodes.cpp:153-204 order):
 //   a = x * chirp (zero-padded to M), A = FFT_M(a), W = FFT_M(conj(A * bhat)),
 //   y[k] = conj(W[k]) * chirp[k].  bhat = FFT_M(b) / M.
 // ---------------------------------------------------------------------------
 template <typename T, int E>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs a) {
+__device__ __forceinline__ void bluestein_body(const FusedArgs &a, int blk) {
   LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
+  const int64_t q0 = static_cast<int64_t>(blk) * a.C;
   const int n = a.n, M = a.m, LS = a.lds_stride;
   const GlbC<T> *chirp = gc<T>(a.chirp);
   const GlbC<T> *bhat = gc<T>(a.bhat);
-  const int s = tid / a.tpu;
-  const int lt = tid - s * a.tpu;
+  int s, lt;
+  slot_lane(tid, a.C, a.tpu, s, lt);
   LdsC<T> *A = lds + s * LS;
   for (int e = n + lt; e < M; e += a.tpu) stc(A + lpad(e), V2<T>{0, 0});
   batched<T, kBatch>(
@@ -512,6 +513,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs 
       });
 }
 
+template <typename T, int E>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs a) {
+  bluestein_body<T, E>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Even-length R2C via an n/2-point complex transform, post-process fused:
 //   z_j = x_2j + i x_2j+1, Z = FFT_h(z), X_k = E_k + w_n^k O_k with
@@ -519,10 +525,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_bluestein(const FusedArgs 
 // Input geometry counts reals (es must be 1 and bases even): z is read as V2.
 // ---------------------------------------------------------------------------
 template <typename T, int E>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a) {
+__device__ __forceinline__ void r2c_even_body(const FusedArgs &a, int blk) {
   LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
+  const int64_t q0 = static_cast<int64_t>(blk) * a.C;
   const int h = a.m, LS = a.lds_stride;
   const GlbC<T> *wn = gc<T>(a.aux);
   batched<T, kBatch>(
@@ -541,8 +547,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a
         stc(lds + s * LS + lpad(e), v);
       });
   __syncthreads();
-  const int s = tid / a.tpu;
-  const int lt = tid - s * a.tpu;
+  int s, lt;
+  slot_lane(tid, a.C, a.tpu, s, lt);
   lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   const T half = T(0.5);
   batched<T, kBatch>(
@@ -569,6 +575,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a
       });
 }
 
+template <typename T, int E>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a) {
+  r2c_even_body<T, E>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Even-length C2R via an n/2-point transform, pre-process fused:
 //   Z'_k = (X_k + conj X_{h-k}) + i w_n^{-k} (X_k - conj X_{h-k}), k < h,
@@ -576,10 +587,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_r2c_even(const FusedArgs a
 // Output geometry counts reals (es == 1, even bases).
 // ---------------------------------------------------------------------------
 template <typename T, int E>
-__global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a) {
+__device__ __forceinline__ void c2r_even_body(const FusedArgs &a, int blk) {
   LdsC<T> *lds = lds_base<T>();
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * a.C;
+  const int64_t q0 = static_cast<int64_t>(blk) * a.C;
   const int h = a.m, LS = a.lds_stride;
   const GlbC<T> *wn = gc<T>(a.aux);
   const GlbC<T> *X = gc<T>(a.in);
@@ -610,8 +621,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a
         stc(lds + s * LS + lpad(k), v);
       });
   __syncthreads();
-  const int s = tid / a.tpu;
-  const int lt = tid - s * a.tpu;
+  int s, lt;
+  slot_lane(tid, a.C, a.tpu, s, lt);
   lds_fft<T, E>(lds + s * LS, h, a.nst, a.rad, a.fd_q, a.twq_step, lt, a.tpu, a.tw_small);
   const T scale = static_cast<T>(a.scale);
   batched<T, kBatch>(
@@ -632,6 +643,51 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a
         const int64_t base = seq_base(static_cast<uint32_t>(q), a, a.go);
         stg((GlbC<T> *)(gr<T>(a.out) + base + 2 * static_cast<int64_t>(j)), z);
       });
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a) {
+  c2r_even_body<T, E>(a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent mixed-length kernel.  A fixed grid of workgroups drains a queue of
+// tiles from single-pass jobs of any length, type and precision: each
+// workgroup takes the next tile index from a global counter, finds its job by
+// binary search over the jobs' first tiles and runs the same device body as
+// the job's standalone kernel (so results are bit-identical to it).  Every
+// workgroup exits once the counter passes the last tile.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kQueueThreads) void k_queue(const QueueArgs q) {
+  __shared__ int item_s;
+  for (;;) {
+    __syncthreads();  // the previous tile is done with LDS and item_s
+    if (threadIdx.x == 0) item_s = static_cast<int>(atomicAdd(q.counter, 1u));
+    __syncthreads();
+    const int item = item_s;
+    if (item >= q.total) break;
+    int lo = 0, hi = q.njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (q.jobs[mid].first <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const QueueJob &j = q.jobs[lo];
+    const int blk = item - j.first;
+    switch (j.kind * 2 + j.dbl) {
+      case kQueuePass * 2: pass_body<float, 16>(j.pa, blk); break;
+      case kQueuePass * 2 + 1: pass_body<double, 8>(j.pa, blk); break;
+      case kQueueRader * 2: rader_body<float, 16>(j.fa, blk); break;
+      case kQueueRader * 2 + 1: rader_body<double, 8>(j.fa, blk); break;
+      case kQueueBluestein * 2: bluestein_body<float, 16>(j.fa, blk); break;
+      case kQueueBluestein * 2 + 1: bluestein_body<double, 8>(j.fa, blk); break;
+      case kQueueR2CEven * 2: r2c_even_body<float, 16>(j.fa, blk); break;
+      case kQueueR2CEven * 2 + 1: r2c_even_body<double, 8>(j.fa, blk); break;
+      case kQueueC2REven * 2: c2r_even_body<float, 16>(j.fa, blk); break;
+      case kQueueC2REven * 2 + 1: c2r_even_body<double, 8>(j.fa, blk); break;
+      default: break;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -747,23 +803,6 @@ __global__ void k_scale_cast(const V2<double> *in, V2<T> *out, int64_t count, do
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename K>
-hipError_t prep_lds(K kernel, size_t lds) {
-  if (lds > 64 * 1024) {
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(lds));
-  }
-  return hipSuccess;
-}
-
-template <typename Args, typename K>
-hipError_t launch_one(K kernel, const Args &a, const LaunchShape &s, hipStream_t st) {
-  hipError_t err = prep_lds(kernel, s.lds_bytes);
-  if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(kernel, s.grid, s.block, s.lds_bytes, st, a);
-  return hipGetLastError();
-}
-
 dim3 grid_for(int64_t total, int threads) {
   int64_t blocks = (total + threads - 1) / threads;
   if (blocks > 65536) blocks = 65536;
@@ -775,48 +814,42 @@ dim3 grid_for(int64_t total, int threads) {
 
 bool supported_ept(bool dbl, int ept) { return dbl ? (ept == 8 || ept == 16) : (ept == 8 || ept == 16 || ept == 32); }
 
-#define FLAGFFT_DISPATCH_EPT(KERNEL, ARGS)                                                  \
-  do {                                                                                      \
-    if (dbl) {                                                                              \
-      if (ept == 8) return launch_one(dev::KERNEL<double, 8>, ARGS, s, st);                 \
-      if (ept == 16) return launch_one(dev::KERNEL<double, 16>, ARGS, s, st);               \
-    } else {                                                                                \
-      if (ept == 8) return launch_one(dev::KERNEL<float, 8>, ARGS, s, st);                  \
-      if (ept == 16) return launch_one(dev::KERNEL<float, 16>, ARGS, s, st);                \
-      if (ept == 32) return launch_one(dev::KERNEL<float, 32>, ARGS, s, st);                \
-    }                                                                                       \
-    return hipErrorInvalidValue;                                                            \
+hipError_t allow_lds(const void *f, size_t bytes) {
+  if (bytes <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+}
+
+#define FLAGFFT_KERNEL_EPT(KERNEL)                                                        \
+  do {                                                                                  \
+    if (dbl) {                                                                          \
+      if (ept == 8) return reinterpret_cast<const void *>(dev::KERNEL<double, 8>);      \
+      if (ept == 16) return reinterpret_cast<const void *>(dev::KERNEL<double, 16>);    \
+    } else {                                                                            \
+      if (ept == 8) return reinterpret_cast<const void *>(dev::KERNEL<float, 8>);       \
+      if (ept == 16) return reinterpret_cast<const void *>(dev::KERNEL<float, 16>);     \
+      if (ept == 32) return reinterpret_cast<const void *>(dev::KERNEL<float, 32>);     \
+    }                                                                                   \
+    return nullptr;                                                                     \
   } while (0)
 
-hipError_t launch_pass(const PassArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st) {
-  FLAGFFT_DISPATCH_EPT(k_pass, a);
-}
-hipError_t launch_rader(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st) {
-  FLAGFFT_DISPATCH_EPT(k_rader, a);
-}
-hipError_t launch_bluestein(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st) {
-  FLAGFFT_DISPATCH_EPT(k_bluestein, a);
-}
-hipError_t launch_r2c_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st) {
-  FLAGFFT_DISPATCH_EPT(k_r2c_even, a);
-}
-hipError_t launch_c2r_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st) {
-  FLAGFFT_DISPATCH_EPT(k_c2r_even, a);
-}
+const void *pass_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_pass); }
+const void *rader_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_rader); }
+const void *bluestein_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_bluestein); }
+const void *r2c_even_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_r2c_even); }
+const void *c2r_even_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_c2r_even); }
 
-hipError_t launch_real_post(const RealPassArgs &a, bool dbl, hipStream_t st) {
-  const int64_t total = a.nseqs * (a.n / 2 + 1);
-  LaunchShape s{grid_for(total, 256), dim3(256), 0};
-  if (dbl) return launch_one(dev::k_real_post<double>, a, s, st);
-  return launch_one(dev::k_real_post<float>, a, s, st);
-}
+const void *queue_kernel() { return reinterpret_cast<const void *>(dev::k_queue); }
 
-hipError_t launch_real_pre(const RealPassArgs &a, bool dbl, hipStream_t st) {
-  const int64_t total = a.nseqs * (a.n / 2);
-  LaunchShape s{grid_for(total, 256), dim3(256), 0};
-  if (dbl) return launch_one(dev::k_real_pre<double>, a, s, st);
-  return launch_one(dev::k_real_pre<float>, a, s, st);
+const void *real_post_kernel(bool dbl) {
+  return dbl ? reinterpret_cast<const void *>(dev::k_real_post<double>)
+             : reinterpret_cast<const void *>(dev::k_real_post<float>);
 }
+const void *real_pre_kernel(bool dbl) {
+  return dbl ? reinterpret_cast<const void *>(dev::k_real_pre<double>)
+             : reinterpret_cast<const void *>(dev::k_real_pre<float>);
+}
+LaunchShape real_post_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 2 + 1), 256), dim3(256), 0}; }
+LaunchShape real_pre_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 2), 256), dim3(256), 0}; }
 
 hipError_t gen_roots(void *out, bool dbl, int64_t count, int64_t n, int64_t mult, hipStream_t st) {
   dim3 g = grid_for(count, 256);

@@ -19,18 +19,39 @@ struct LaunchShape {
 bool supported_ept(bool dbl, int ept);  // fp32: 8, 16, 32; fp64: 8, 16
 constexpr int kMaxBlockThreads = 512;
 
-hipError_t launch_pass(const PassArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
-hipError_t launch_rader(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
-hipError_t launch_bluestein(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
-hipError_t launch_r2c_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
-hipError_t launch_c2r_even(const FusedArgs &a, bool dbl, int ept, const LaunchShape &s, hipStream_t st);
+// Host handles of the kernels every plan step launches (nullptr when no such
+// instance is compiled).  A step is launched with hipLaunchKernel(handle,
+// grid, block, {&args}, lds, stream), its only argument being its args struct.
+const void *pass_kernel(bool dbl, int ept);
+const void *rader_kernel(bool dbl, int ept);
+const void *bluestein_kernel(bool dbl, int ept);
+const void *r2c_even_kernel(bool dbl, int ept);
+const void *c2r_even_kernel(bool dbl, int ept);
+const void *real_post_kernel(bool dbl);
+const void *real_pre_kernel(bool dbl);
+LaunchShape real_post_shape(const RealPassArgs &a);
+LaunchShape real_pre_shape(const RealPassArgs &a);
+
 // Specialised single-pass kernels: id of the compiled radix sequence (-1 if none).
 int leaf_kernel_id(const std::vector<int> &radices);
 int leaf_kernel_ept(int id);
-hipError_t launch_leaf(const LeafArgs &a, bool dbl, int id, const LaunchShape &s, hipStream_t st);
+const void *leaf_kernel(bool dbl, int id);
 
-hipError_t launch_real_post(const RealPassArgs &a, bool dbl, hipStream_t st);
-hipError_t launch_real_pre(const RealPassArgs &a, bool dbl, hipStream_t st);
+// Specialised super-stage kernels: id of the compiled radix sequence (-1 if none),
+// its units per workgroup and elements per thread.
+int stage_kernel_id(const std::vector<int> &radices);
+void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept);
+const void *stage_kernel(bool dbl, int id);
+
+// Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
+// QueueArgs argument).
+const void *queue_kernel();
+
+// Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).
+const void *mfma_kernel(int mode);
+
+// Raise a kernel's dynamic-LDS limit when a launch needs more than 64 KiB.
+hipError_t allow_lds(const void *f, size_t bytes);
 
 // ---- table generation on the GPU, FP64 arithmetic ------------------------
 // out[j] = w_n^{(j*mult) mod n} = exp(-2 pi i ((j*mult) mod n) / n), j < count.

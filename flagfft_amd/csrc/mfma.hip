@@ -1,0 +1,208 @@
+// mfma.hip — split-precision matrix-core transform of length 1024 = 32 x 32 (gfx950).
+//
+// x[32 n1 + n2] is viewed as a 32x32 matrix.  The two radix-32 DFT stages are
+// dense complex matrix products on v_mfma_f32_32x32x16_{bf16,f16}:
+//
+//   step 1   B^T[n2][k1] = sum_n1 x[32 n1 + n2] F[n1][k1]      (data = A operand)
+//   twiddle  B^T[n2][k1] *= w_1024^(n2 k1)                    (FP32, in the accumulators)
+//   step 2   C^T[k2][k1] = sum_n2 F[k2][n2] B^T[n2][k1]         (data = B operand)
+//   out      X[k1 + 32 k2] = C^T[k2][k1]
+//
+// with F[a][b] = w_32^(ab).  Complex products are real products on stacked
+// real/imaginary parts.  Every FP32 operand v is split into v_hi = half(v) and
+// v_lo = half(v - v_hi); each real product keeps hi*hi + hi*lo + lo*hi (three
+// MFMAs, FP32 accumulation), dropping only lo*lo.
+//
+// Lane layout of the 32x32x16 instructions (wave64, lane l, c = l % 32,
+// h = l / 32): A operand A[c][8h + v], B operand B[8h + v][c] (v < 8),
+// accumulator register r holds D[8(r/4) + 4h + r%4][c].  The accumulator of
+// step 1 is reused as step 2's B operand without any data movement: chunk q
+// of the K loop pairs lane group h, element v with n2 = 16q + 8(v/4) + 4h + v%4,
+// which is accumulator register 8q + v, and step 2's F operand is permuted the
+// same way.  Loads (x[32 n1 + c]) and stores (X[c + 32 k2]) are 256-byte row
+// segments per half wave.
+//
+// FP16 has a narrow exponent range, so each transform is scaled by a power of
+// two (exact) that brings max |x| below 1 before the split, and scaled back on
+// store.
+#include <hip/hip_runtime.h>
+
+#include "dev_common.hpp"
+#include "fft_args.hpp"
+#include "kernels.hpp"
+
+namespace flagfft {
+namespace dev {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <bool F16>
+struct HalfT;
+template <>
+struct HalfT<false> {
+  using E = __bf16;
+  using V8 = bf16x8;
+};
+template <>
+struct HalfT<true> {
+  using E = _Float16;
+  using V8 = f16x8;
+};
+
+template <bool F16>
+struct Split8 {
+  typename HalfT<F16>::V8 hi, lo;
+};
+
+template <bool F16>
+__device__ __forceinline__ f32x16 mma(typename HalfT<F16>::V8 a, typename HalfT<F16>::V8 b, f32x16 c) {
+  if constexpr (F16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// acc += a * b with both operands split: hi*hi + hi*lo + lo*hi.
+template <bool F16>
+__device__ __forceinline__ f32x16 mma3(const Split8<F16> &a, const Split8<F16> &b, f32x16 acc) {
+  acc = mma<F16>(a.lo, b.hi, acc);
+  acc = mma<F16>(a.hi, b.lo, acc);
+  return mma<F16>(a.hi, b.hi, acc);
+}
+
+// element v of s := split of x
+template <bool F16>
+__device__ __forceinline__ void split_into(Split8<F16> &s, int v, float x) {
+  using E = typename HalfT<F16>::E;
+  const E hi = static_cast<E>(x);
+  s.hi[v] = hi;
+  s.lo[v] = static_cast<E>(x - static_cast<float>(hi));
+}
+
+template <bool F16>
+__device__ __forceinline__ Split8<F16> neg(const Split8<F16> &s) {
+  return Split8<F16>{-s.hi, -s.lo};
+}
+
+// w_1024^e from the FP32 table (e reduced mod 1024).
+__device__ __forceinline__ V2<float> w1024(const GlbC<float> *tw, int e) { return ldg(tw + (e & 1023)); }
+
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mfma1024(const MfmaArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 31;
+  const int h = lane >> 5;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+  const GlbC<float> *tw = gc<float>(a.tw);
+
+  // per-lane constant operands (identical for every transform this wave handles)
+  Split8<F16> fb_r[2], fb_i[2];  // step 1 B operand: F[16q + 8h + v][c]
+  Split8<F16> fa_r[2], fa_i[2];  // step 2 A operand: F[c][16q + 8(v/4) + 4h + v%4]
+  V2<float> twid[16];            // w_1024^(n2 c), n2 = 8(r/4) + 4h + r%4
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const V2<float> f1 = w1024(tw, 32 * (16 * q + 8 * h + v) * c);
+      const V2<float> f2 = w1024(tw, 32 * c * (16 * q + 8 * (v >> 2) + 4 * h + (v & 3)));
+      split_into<F16>(fb_r[q], v, f1.x);
+      split_into<F16>(fb_i[q], v, f1.y);
+      split_into<F16>(fa_r[q], v, f2.x);
+      split_into<F16>(fa_i[q], v, f2.y);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) twid[r] = w1024(tw, (8 * (r >> 2) + 4 * h + (r & 3)) * c);
+
+  const float cin = a.conj_in ? -1.f : 1.f;
+  const float cout = a.conj_out ? -1.f : 1.f;
+  const float user_scale = static_cast<float>(a.scale);
+
+  for (int64_t seq = wave; seq < a.nseq; seq += nwaves) {
+    const GlbC<float> *src = gc<float>(a.in) + seq * a.idist + c;
+    GlbC<float> *dst = gc<float>(a.out) + seq * a.odist + c;
+    V2<float> x[2][8];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) x[q][v] = ldg(src + 32 * (16 * q + 8 * h + v));
+
+    // exact power-of-two range scaling (FP16 only)
+    float pre = 1.f, post = 1.f;
+    if constexpr (F16) {
+      float m = 0.f;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) m = fmaxf(m, fmaxf(fabsf(x[q][v].x), fabsf(x[q][v].y)));
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      if (m > 0.f && m == m && m <= 3.0e38f) {
+        int e = 0;
+        (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+        pre = ldexpf(1.f, -e);
+        post = ldexpf(1.f, e);
+      }
+    }
+
+    Split8<F16> ar[2], ai[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        split_into<F16>(ar[q], v, x[q][v].x * pre);
+        split_into<F16>(ai[q], v, cin * x[q][v].y * pre);
+      }
+
+    // step 1: B^T = X^T F  (re: Ar Fr - Ai Fi, im: Ar Fi + Ai Fr)
+    f32x16 br = {}, bi = {};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      br = mma3<F16>(ar[q], fb_r[q], br);
+      br = mma3<F16>(neg<F16>(ai[q]), fb_i[q], br);
+      bi = mma3<F16>(ar[q], fb_i[q], bi);
+      bi = mma3<F16>(ai[q], fb_r[q], bi);
+    }
+
+    // twiddle in FP32, then split into step 2's B operand (register 8q + v)
+    Split8<F16> sr[2], si[2];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const V2<float> t = twid[r];
+      const float re = br[r] * t.x - bi[r] * t.y;
+      const float im = br[r] * t.y + bi[r] * t.x;
+      split_into<F16>(sr[r >> 3], r & 7, re);
+      split_into<F16>(si[r >> 3], r & 7, im);
+    }
+
+    // step 2: C^T = F B^T  (re: Fr Br - Fi Bi, im: Fr Bi + Fi Br)
+    f32x16 cr = {}, ci = {};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      cr = mma3<F16>(fa_r[q], sr[q], cr);
+      cr = mma3<F16>(fa_i[q], neg<F16>(si[q]), cr);
+      ci = mma3<F16>(fa_r[q], si[q], ci);
+      ci = mma3<F16>(fa_i[q], sr[q], ci);
+    }
+
+    const float so = post * user_scale;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int k2 = 8 * (r >> 2) + 4 * h + (r & 3);
+      stg(dst + 32 * k2, V2<float>{cr[r] * so, cout * ci[r] * so});
+    }
+  }
+}
+
+}  // namespace dev
+
+const void *mfma_kernel(int mode) {
+  if (mode == kComputeFp16x3) return reinterpret_cast<const void *>(dev::k_mfma1024<true>);
+  if (mode == kComputeBf16x3) return reinterpret_cast<const void *>(dev::k_mfma1024<false>);
+  return nullptr;
+}
+
+}  // namespace flagfft

@@ -102,38 +102,55 @@ bool radix_factor(int64_t n, std::vector<int> &out) {
   return true;
 }
 
+bool passes_specialised(const std::vector<int64_t> &passes) {
+  std::vector<int> rad;
+  for (int64_t r : passes)
+    if (!radix_factor(r, rad) || stage_kernel_id(rad) < 0) return false;
+  return true;
+}
+
 int64_t leaf_max(const PlanOptions &o, Prec p) { return p == Prec::F32 ? o.leaf_max_f32 : o.leaf_max_f64; }
 int64_t pass_max(const PlanOptions &o, Prec p) { return p == Prec::F32 ? o.pass_max_f32 : o.pass_max_f64; }
 
 namespace {
 
-// Split n into k factors, each <= cap and radix-factorable; maximise the smallest factor.
-bool split_passes(int64_t n, int k, int64_t cap, std::vector<int64_t> &best) {
+// All splits of n into k factors (non-increasing), each <= cap and radix-factorable.
+void enum_splits(int64_t n, int k, int64_t cap, int64_t hi, std::vector<int64_t> &cur,
+                 std::vector<std::vector<int64_t>> &out) {
   std::vector<int> tmp;
   if (k == 1) {
-    if (n <= cap && radix_factor(n, tmp)) {
-      best = {n};
-      return true;
+    if (n <= cap && n <= hi && n >= 2 && radix_factor(n, tmp)) {
+      cur.push_back(n);
+      out.push_back(cur);
+      cur.pop_back();
     }
-    return false;
+    return;
   }
-  bool found = false;
-  int64_t best_min = 0;
-  for (int64_t d = 2; d <= cap && d <= n; ++d) {
+  for (int64_t d = std::min(cap, hi); d >= 2; --d) {
     if (n % d != 0 || !radix_factor(d, tmp)) continue;
-    std::vector<int64_t> rest;
-    if (!split_passes(n / d, k - 1, cap, rest)) continue;
-    int64_t mn = std::min(d, *std::min_element(rest.begin(), rest.end()));
-    if (!found || mn > best_min) {
-      found = true;
-      best_min = mn;
-      best.clear();
-      best.push_back(d);
-      best.insert(best.end(), rest.begin(), rest.end());
-    }
+    cur.push_back(d);
+    enum_splits(n / d, k - 1, cap, d, cur, out);
+    cur.pop_back();
   }
-  if (found) std::sort(best.begin(), best.end(), std::greater<int64_t>());
-  return found;
+}
+
+// Split n into k passes.  Prefer splits whose every pass has a specialised
+// kernel, then the one with the largest smallest factor (fewest HBM round
+// trips per element are equal for a fixed k; balance keeps every pass's
+// workgroups full).
+bool split_passes(int64_t n, int k, int64_t cap, std::vector<int64_t> &best) {
+  std::vector<std::vector<int64_t>> all;
+  std::vector<int64_t> cur;
+  enum_splits(n, k, cap, cap, cur, all);
+  if (all.empty()) return false;
+  auto score = [](const std::vector<int64_t> &v) {
+    return std::make_pair(passes_specialised(v) ? 1 : 0, *std::min_element(v.begin(), v.end()));
+  };
+  const std::vector<int64_t> *b = &all[0];
+  for (const auto &v : all)
+    if (score(v) > score(*b)) b = &v;
+  best = *b;
+  return true;
 }
 
 bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c) {
@@ -204,7 +221,9 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
     }
   }
   if (smooth13(n) && multipass_core(n, p, o, c)) return c;
-  // Bluestein: smallest 13-smooth convolution length M >= 2n-1 that plans.
+  // Bluestein: smallest 13-smooth convolution length M >= 2n-1 that plans; a
+  // multi-pass M up to 25% longer whose passes all have specialised kernels
+  // is preferred (those run 2-3x faster than the generic pass).
   for (int64_t M = 2 * n - 1;; ++M) {
     if (!smooth13(M)) continue;
     if (M <= lmax && radix_factor(M, rad)) {
@@ -216,6 +235,16 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
     }
     Core sub;
     if (multipass_core(M, p, o, sub)) {
+      if (o.forced_passes.empty() && !passes_specialised(sub.passes)) {
+        for (int64_t M2 = M + 1; M2 <= M + M / 4; ++M2) {
+          Core sub2;
+          if (smooth13(M2) && multipass_core(M2, p, o, sub2) && passes_specialised(sub2.passes)) {
+            M = M2;
+            sub = sub2;
+            break;
+          }
+        }
+      }
       c.kind = CoreKind::Bluestein;
       c.m = M;
       c.radices.clear();
@@ -242,6 +271,10 @@ std::string Core::describe(int indent) const {
     case CoreKind::Rader:
       o << pad << "Rader(p=" << n << ", root=" << root << ", conv_length=" << m << ", fused=1)\n"
         << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
+      break;
+    case CoreKind::MatrixCore:
+      o << pad << "MatrixCoreSplit(n=" << n << ", factors=[" << join(radices, ",") << "], operands="
+        << (compute == 2 ? "fp16" : "bf16") << " hi+lo, products=3, accumulate=fp32)";
       break;
     case CoreKind::Bluestein:
       o << pad << "Bluestein(n=" << n << ", conv_length=" << m << ", fused=" << (conv ? 0 : 1) << ")\n";
@@ -302,6 +335,17 @@ Plan1D plan_1d(int64_t n, bool real_fwd, bool real_inv, Prec p, const PlanOption
 PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
   PlanTree t;
   t.desc = d;
+  if (d.compute != 0) {
+    if (d.rank != 1 || d.real_fwd || d.real_inv || d.prec != Prec::F32 || d.n0 != 1024)
+      throw std::invalid_argument("matrix-core compute modes cover C2C n=1024 only");
+    t.rows.n = d.n0;
+    t.rows.real_form = RealForm::None;
+    t.rows.core.kind = CoreKind::MatrixCore;
+    t.rows.core.n = d.n0;
+    t.rows.core.radices = {32, 32};
+    t.rows.core.compute = d.compute;
+    return t;
+  }
   if (d.rank == 1) {
     t.rows = plan_1d(d.n0, d.real_fwd, d.real_inv, d.prec, o);
   } else {

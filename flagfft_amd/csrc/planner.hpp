@@ -34,9 +34,11 @@ struct PlanOptions {
   std::vector<int64_t> forced_passes;  // explicit super-radix split (tuned plans)
   std::vector<int> forced_radices;     // explicit single-pass radix order (tuned plans)
   int forced_units = 0;                // explicit units per workgroup (tuned plans)
+  int max_threads = 512;               // workgroup size limit of the launches
+  bool generic_kernels = false;        // only the run-time-radix kernels (persistent-queue jobs)
 };
 
-enum class CoreKind { Leaf, MultiPass, Rader, Bluestein };
+enum class CoreKind { Leaf, MultiPass, Rader, Bluestein, MatrixCore };
 
 // A length-n complex transform of a batch of sequences.
 struct Core {
@@ -48,6 +50,7 @@ struct Core {
   int64_t m = 0;                             // Rader: p-1; Bluestein: M
   int64_t root = 0;                          // Rader primitive root
   std::shared_ptr<Core> conv;                // Bluestein: multi-pass core of M (null when fused)
+  int compute = 0;                           // MatrixCore: 1 = BF16 hi/lo, 2 = FP16 hi/lo operands
   std::string describe(int indent = 0) const;
 };
 
@@ -68,6 +71,7 @@ struct PlanDesc {
   int64_t batch = 1;
   Prec prec = Prec::F32;
   bool real_fwd = false, real_inv = false;
+  int compute = 0;             // 0 = vector FP32/FP64; 1/2 = split BF16/FP16 matrix cores (C2C n=1024)
 };
 
 struct PlanTree {
@@ -85,6 +89,12 @@ bool smooth13(int64_t n);
 int64_t leaf_max(const PlanOptions &o, Prec p);
 int64_t pass_max(const PlanOptions &o, Prec p);
 inline int lds_pad(int e) { return e + (e >> 5); }   // must match the kernels' lpad()
+
+// Catalogue of the compile-time specialised super-stage kernels (leaf.hip):
+// id of the kernel for this radix sequence, -1 when none is compiled.
+int stage_kernel_id(const std::vector<int> &radices);
+// True when every super-radix of the split has a specialised kernel.
+bool passes_specialised(const std::vector<int64_t> &passes);
 
 Core plan_core(int64_t n, Prec p, const PlanOptions &o);
 PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o);

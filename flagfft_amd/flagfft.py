@@ -52,7 +52,11 @@ SYMBOLS = [
     "flagfftSetStream", "flagfftDestroy", "flagfftGetPlanDescription",
 ]
 EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning",
-               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel"]
+               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute",
+               "flagfftxSetGraphMode"]
+
+# flagfftxCompute (include/flagfft_ext.h)
+COMPUTE_VECTOR, COMPUTE_BF16X3, COMPUTE_FP16X3 = 0, 1, 2
 
 
 class FlagFFTError(RuntimeError):
@@ -103,6 +107,8 @@ def lib():
             "flagfftDestroy": [_H],
             "flagfftxPlanDescribe": [_int, _IP, _int, _int, ctypes.c_char_p, ctypes.c_size_t],
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
+            "flagfftxPlan1dCompute": [_HP, _int, _int, _int, _int],
+            "flagfftxSetGraphMode": [_H, _int],
             "flagfftxSetTuning": [ctypes.POINTER(Tuning)],
             "flagfftxSetProfiling": [_H, _int],
             "flagfftxGetProfile": [_H, _int, _int, ctypes.POINTER(ctypes.c_double),
@@ -132,6 +138,12 @@ def _arr(values):
 def plan1d(nx: int, type_: int, batch: int):
     h = ctypes.c_void_p()
     r = lib().flagfftPlan1d(ctypes.byref(h), nx, int(type_), batch)
+    return r, h
+
+
+def plan1d_compute(nx: int, type_: int, batch: int, compute: int):
+    h = ctypes.c_void_p()
+    r = lib().flagfftxPlan1dCompute(ctypes.byref(h), nx, int(type_), batch, int(compute))
     return r, h
 
 
@@ -269,6 +281,12 @@ class Plan:
         return cls(h)
 
     @classmethod
+    def one_d_compute(cls, n: int, type_: int, batch: int, compute: int):
+        r, h = plan1d_compute(n, type_, batch, compute)
+        check("flagfftxPlan1dCompute", r)
+        return cls(h)
+
+    @classmethod
     def two_d(cls, nx: int, ny: int, type_: int):
         r, h = plan2d(nx, ny, type_)
         check("flagfftPlan2d", r)
@@ -279,6 +297,9 @@ class Plan:
         r, h = plan_many(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch)
         check("flagfftPlanMany", r)
         return cls(h)
+
+    def set_graph_mode(self, enable: bool = True):
+        check("flagfftxSetGraphMode", lib().flagfftxSetGraphMode(self.handle, int(bool(enable))))
 
     def set_stream(self, stream: int):
         check("flagfftSetStream", set_stream(self.handle, stream))

@@ -60,6 +60,29 @@ typedef struct flagfftxTuning_t {
 
 flagfftResult flagfftxSetTuning(const flagfftxTuning *tuning); /* NULL restores defaults */
 
+/* Graph mode: each Exec replays the plan's kernel launches as one hipGraph
+ * (recorded at the first Exec of each direction; later Execs patch only the
+ * nodes that touch the caller's buffers).  Results are identical to the
+ * default mode.  While profiling is enabled Exec launches directly. */
+flagfftResult flagfftxSetGraphMode(flagfftHandle plan, int enable);
+
+/* Arithmetic of a plan.  VECTOR is the FP32/FP64 path of flagfftPlan1d.  The
+ * split-precision modes run the two radix-32 stages of a length-1024 C2C
+ * transform as dense complex matrix products on the matrix cores: every FP32
+ * operand is split into hi + lo halves (BF16 or FP16), three products per real
+ * product (hi*hi + hi*lo + lo*hi) accumulate in FP32. */
+typedef enum flagfftxCompute_t {
+  FLAGFFTX_COMPUTE_VECTOR = 0,
+  FLAGFFTX_COMPUTE_BF16X3 = 1,
+  FLAGFFTX_COMPUTE_FP16X3 = 2
+} flagfftxCompute;
+
+/* flagfftPlan1d with an arithmetic mode.  The split-precision modes accept
+ * FLAGFFT_C2C with nx == 1024 only (FLAGFFT_NOT_SUPPORTED otherwise); an
+ * unknown mode is FLAGFFT_INVALID_VALUE.  Exec, SetStream and Destroy are the
+ * usual calls. */
+flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType type, int batch, int compute);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,24 @@ def test_valid_descriptors_reach_the_device_check():
     assert ff.plan2d(64, 32, T.R2C)[0] == R.INVALID_DEVICE
 
 
+def test_compute_mode_validation():
+    """flagfftxPlan1dCompute: unknown mode, unsupported type / length, device check."""
+    import torch
+
+    assert ff.lib().flagfftxPlan1dCompute(None, 1024, int(T.C2C), 1, 1) == R.INVALID_VALUE
+    assert ff.plan1d_compute(1024, T.C2C, 1, 3)[0] == R.INVALID_VALUE
+    assert ff.plan1d_compute(1024, T.C2C, 1, -1)[0] == R.INVALID_VALUE
+    for compute in (ff.COMPUTE_BF16X3, ff.COMPUTE_FP16X3):
+        assert ff.plan1d_compute(2048, T.C2C, 1, compute)[0] == R.NOT_SUPPORTED
+        assert ff.plan1d_compute(1024, T.Z2Z, 1, compute)[0] == R.NOT_SUPPORTED
+        assert ff.plan1d_compute(1024, T.R2C, 1, compute)[0] == R.NOT_SUPPORTED
+        assert ff.plan1d_compute(0, T.C2C, 1, compute)[0] == R.INVALID_SIZE
+        assert ff.plan1d_compute(1024, T.C2C, 0, compute)[0] == R.INVALID_VALUE
+        if not torch.cuda.is_available():
+            r, h = ff.plan1d_compute(1024, T.C2C, 8, compute)
+            assert r == R.INVALID_DEVICE and not h.value
+
+
 def test_null_handles():
     L = ff.lib()
     assert L.flagfftDestroy(None) == R.INVALID_PLAN

@@ -1,0 +1,99 @@
+"""Split-precision matrix-core C2C (N=1024) against the FP64 oracle (GPU).
+
+The radix-32 x 32 stages run as BF16 / FP16 hi+lo matrix products with FP32
+accumulation (flagfftxPlan1dCompute).  Each test states the error it measures
+against the FP64 oracle next to the plain FP32 vector path on the same data.
+
+Bounds (relative L2 against the FP64 oracle, N=1024):
+  FP16 hi+lo: 2e-6 * log2 N = 2e-5, the stated FP32 bound (the split keeps 22
+              significand bits, the dropped lo*lo term is ~2^-22 relative);
+  BF16 hi+lo: 4e-5 (16 significand bits, dropped term ~2^-16 relative).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flagfft_amd import flagfft as ff
+from oracle import oracle as ora
+
+from .helpers import complex_input, dev_tensor, host, stream_of, sync
+
+pytestmark = pytest.mark.gpu
+
+N = 1024
+BOUND = {ff.COMPUTE_FP16X3: 2e-6 * math.log2(N), ff.COMPUTE_BF16X3: 4e-5}
+NAMES = {ff.COMPUTE_FP16X3: "fp16x3", ff.COMPUTE_BF16X3: "bf16x3"}
+
+
+def _run(dev, x, direction, compute):
+    batch = x.shape[0]
+    xd = dev_tensor(x, dev)
+    yd = xd.clone().zero_()
+    with ff.Plan.one_d_compute(N, ff.Type.C2C, batch, compute) as p:
+        p.set_stream(stream_of(dev))
+        p.c2c(xd.data_ptr(), yd.data_ptr(), direction)
+        sync(dev)
+        desc = p.description()
+    return host(yd), desc
+
+
+@pytest.mark.parametrize("compute", [ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3], ids=["fp16x3", "bf16x3"])
+@pytest.mark.parametrize("batch", [1, 7, 256])
+@pytest.mark.parametrize("direction", [ff.FORWARD, ff.INVERSE], ids=["fwd", "inv"])
+def test_split_precision_matches_oracle(gpu, compute, batch, direction):
+    seed = ora.accuracy_seed(int(ff.Type.C2C), N, batch, 7)
+    x = complex_input(seed, (batch, N), np.complex64, "normal")
+    y, desc = _run(gpu, x, direction, compute)
+    assert "MatrixCoreSplit(n=1024" in desc and "k_mfma1024" in desc
+    ref = ora.fft(x.astype(np.complex128), sign=direction)
+    l2, linf = ora.rel_l2(y, ref, N)
+    print(f"{NAMES[compute]} batch={batch} dir={direction}: rel_l2={l2:.3e} rel_linf={linf:.3e} "
+          f"(bound {BOUND[compute]:.1e})")
+    assert np.isfinite(l2) and l2 <= BOUND[compute]
+
+
+@pytest.mark.parametrize("compute", [ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3], ids=["fp16x3", "bf16x3"])
+@pytest.mark.parametrize("scale", [2.0 ** -20, 2.0 ** 20, 1e30])
+def test_split_precision_input_scales(gpu, compute, scale):
+    """Per-transform power-of-two range scaling keeps FP16 usable at any magnitude."""
+    x = complex_input(ora.accuracy_seed(int(ff.Type.C2C), N, 4, 8), (4, N), np.complex64, "normal", scale)
+    y, _ = _run(gpu, x, ff.FORWARD, compute)
+    ref = ora.fft(x.astype(np.complex128))
+    l2, _ = ora.rel_l2(y, ref, N)
+    assert np.isfinite(l2) and l2 <= BOUND[compute], f"scale={scale} rel_l2={l2:.3e}"
+
+
+@pytest.mark.parametrize("compute", [ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3], ids=["fp16x3", "bf16x3"])
+def test_split_precision_closed_forms(gpu, compute):
+    """Impulse at j -> w^(jk); constant -> N delta_0; a pure tone -> N delta_m."""
+    rows = []
+    for j in (0, 1, 33, 1023):
+        e = np.zeros(N, np.complex64)
+        e[j] = 1
+        rows.append(e)
+    rows.append(np.ones(N, np.complex64))
+    rows.append(np.exp(2j * np.pi * 77 * np.arange(N) / N).astype(np.complex64))
+    x = np.stack(rows)
+    y, _ = _run(gpu, x, ff.FORWARD, compute)
+    k = np.arange(N)
+    expect = [np.exp(-2j * np.pi * j * k / N) for j in (0, 1, 33, 1023)]
+    expect.append(N * (k == 0))
+    expect.append(N * (k == 77))
+    for got, want in zip(y, expect):
+        assert np.linalg.norm(got - want) / np.linalg.norm(want) <= BOUND[compute]
+
+
+def test_split_precision_same_data_as_vector_path(gpu):
+    """Error of each path on the same data (the report the precision form asks for)."""
+    batch = 512
+    x = complex_input(ora.accuracy_seed(int(ff.Type.C2C), N, batch, 9), (batch, N), np.complex64, "normal")
+    ref = ora.fft(x.astype(np.complex128))
+    errs = {}
+    for compute in (ff.COMPUTE_VECTOR, ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3):
+        y, _ = _run(gpu, x, ff.FORWARD, compute)
+        errs[compute] = ora.rel_l2(y, ref, N)[0]
+    print("rel_l2 vs FP64 oracle: " + ", ".join(f"{k}={v:.3e}" for k, v in errs.items()))
+    assert errs[ff.COMPUTE_VECTOR] <= 2e-6 * math.log2(N)
+    assert errs[ff.COMPUTE_FP16X3] <= BOUND[ff.COMPUTE_FP16X3]
+    assert errs[ff.COMPUTE_BF16X3] <= BOUND[ff.COMPUTE_BF16X3]
