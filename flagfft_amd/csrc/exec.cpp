@@ -1071,7 +1071,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
 }
 
 std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o, int64_t idist,
-                                             int64_t odist) {
+                                             int64_t odist, bool allocate_ws) {
   auto ex = std::make_unique<Executable>();
   Builder b(t.desc.prec, o, *ex, nullptr);
   const int64_t batch = t.desc.batch;
@@ -1129,7 +1129,7 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
   b.finish();
   ex->built_batch = t.desc.rank == 1 ? t.desc.batch : 1;
   ex->resolve_kernels();
-  ex->allocate_workspaces();
+  if (allocate_ws) ex->allocate_workspaces();
   return ex;
 }
 

@@ -165,7 +165,8 @@ class Builder {
 };
 
 // Whole-plan builders (forward / inverse executables of a PlanTree).
+// allocate_ws = false leaves the workspaces to the caller (Executable::run_batch).
 std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o,
-                                             int64_t idist, int64_t odist);
+                                             int64_t idist, int64_t odist, bool allocate_ws = true);
 
 }  // namespace flagfft

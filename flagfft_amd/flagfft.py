@@ -53,7 +53,11 @@ SYMBOLS = [
 ]
 EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning",
                "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute",
-               "flagfftxSetGraphMode"]
+               "flagfftxSetGraphMode", "flagfftxServerCreate", "flagfftxServerPrepare", "flagfftxServerSubmit",
+               "flagfftxServerPoll", "flagfftxServerWait", "flagfftxServerRelease", "flagfftxServerGetStats",
+               "flagfftxServerResetStats", "flagfftxServerDestroy"]
+
+SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
 # flagfftxCompute (include/flagfft_ext.h)
 COMPUTE_VECTOR, COMPUTE_BF16X3, COMPUTE_FP16X3 = 0, 1, 2
@@ -75,6 +79,19 @@ class Tuning(ctypes.Structure):
         ("chunk_bytes", ctypes.c_int), ("npasses", ctypes.c_int), ("passes", ctypes.c_int * 4),
         ("nradices", ctypes.c_int), ("radices", ctypes.c_int * 16),
     ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_longlong), ("transforms", ctypes.c_longlong), ("bytes", ctypes.c_double),
+        ("span_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("batches", ctypes.c_longlong),
+        ("queue_launches", ctypes.c_longlong), ("queued_requests", ctypes.c_longlong),
+        ("arena_peak_bytes", ctypes.c_double), ("table_bytes", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -109,6 +126,17 @@ def lib():
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
             "flagfftxPlan1dCompute": [_HP, _int, _int, _int, _int],
             "flagfftxSetGraphMode": [_H, _int],
+            "flagfftxServerCreate": [_HP, ctypes.c_size_t, _int, _int],
+            "flagfftxServerPrepare": [_H, _int, _int, _int],
+            "flagfftxServerSubmit": [_H, _int, _int, _int, _int, _vp, ctypes.POINTER(ctypes.c_longlong)],
+            "flagfftxServerPoll": [_H, ctypes.c_longlong, _IP, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(ctypes.c_double)],
+            "flagfftxServerWait": [_H, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(ctypes.c_double)],
+            "flagfftxServerRelease": [_H, ctypes.c_longlong],
+            "flagfftxServerGetStats": [_H, ctypes.POINTER(ServerStats)],
+            "flagfftxServerResetStats": [_H],
+            "flagfftxServerDestroy": [_H],
             "flagfftxSetTuning": [ctypes.POINTER(Tuning)],
             "flagfftxSetProfiling": [_H, _int],
             "flagfftxGetProfile": [_H, _int, _int, ctypes.POINTER(ctypes.c_double),
@@ -347,3 +375,61 @@ class Plan:
                 destroy(self.handle)
         except Exception:
             pass
+
+
+class Server:
+    """Owning wrapper of a flagfftxServer (the serving loop of include/flagfft_ext.h)."""
+
+    def __init__(self, budget_bytes: int, max_inflight: int = 3, mode: int = SERVE_BATCHED):
+        h = ctypes.c_void_p()
+        check("flagfftxServerCreate", lib().flagfftxServerCreate(ctypes.byref(h), budget_bytes, max_inflight, mode))
+        self.handle = h
+
+    def prepare(self, type_: int, n: int, direction: int = FORWARD):
+        check("flagfftxServerPrepare", lib().flagfftxServerPrepare(self.handle, int(type_), n, direction))
+
+    def submit(self, type_: int, n: int, batch: int, idata: int, direction: int = FORWARD) -> int:
+        t = ctypes.c_longlong()
+        check("flagfftxServerSubmit",
+              lib().flagfftxServerSubmit(self.handle, int(type_), n, batch, direction, idata, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int):
+        """(done, output device pointer or None, latency ms)."""
+        d = ctypes.c_int()
+        o = ctypes.c_void_p()
+        l = ctypes.c_double()
+        check("flagfftxServerPoll", lib().flagfftxServerPoll(self.handle, ticket, ctypes.byref(d), ctypes.byref(o),
+                                                            ctypes.byref(l)))
+        return bool(d.value), o.value, l.value
+
+    def wait(self, ticket: int):
+        """(output device pointer, latency ms), blocking."""
+        o = ctypes.c_void_p()
+        l = ctypes.c_double()
+        check("flagfftxServerWait", lib().flagfftxServerWait(self.handle, ticket, ctypes.byref(o), ctypes.byref(l)))
+        return o.value, l.value
+
+    def release(self, ticket: int):
+        check("flagfftxServerRelease", lib().flagfftxServerRelease(self.handle, ticket))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check("flagfftxServerGetStats", lib().flagfftxServerGetStats(self.handle, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        check("flagfftxServerResetStats", lib().flagfftxServerResetStats(self.handle))
+
+    def close(self):
+        if self.handle:
+            r = lib().flagfftxServerDestroy(self.handle)
+            self.handle = None
+            check("flagfftxServerDestroy", r)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False

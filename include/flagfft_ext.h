@@ -83,6 +83,56 @@ typedef enum flagfftxCompute_t {
  * usual calls. */
 flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType type, int batch, int compute);
 
+/* ---- serving loop ---------------------------------------------------------
+ * A server owns a device arena of `hbm_budget_bytes` for request outputs and
+ * workspaces, a cache of plans and twiddle tables resident on the device, and
+ * a dispatcher thread.  Submit enqueues a request (device input pointer; the
+ * input must stay valid until the request completes) and returns at once.  A
+ * request is admitted when its output and workspace fit in the arena (FIFO),
+ * so requests that arrive while others run start as memory frees up.
+ *   BATCHED: admitted requests are formed into batches, one per free stream
+ *     slot (up to max_inflight batches overlap on the device).  Single-pass
+ *     requests of a batch run together in one persistent mixed-length kernel
+ *     launch that drains a tile queue; longer or larger requests run their own
+ *     plan's kernels in the same batch.
+ *   PLAIN: each admitted request is launched on its own, in arrival order, on
+ *     one stream, with its cached plan (what per-request Exec calls do).
+ * The output stays in the arena until Release.  Latency is measured from
+ * Submit to the completion of the request's last kernel, as seen by the
+ * dispatcher. */
+typedef struct flagfftxServer_st *flagfftxServer;
+
+typedef enum flagfftxServeMode_t { FLAGFFTX_SERVE_BATCHED = 0, FLAGFFTX_SERVE_PLAIN = 1 } flagfftxServeMode;
+
+typedef struct flagfftxServerStats_t {
+  long long completed;       /* requests completed since the last reset */
+  long long transforms;      /* sum of their batch counts */
+  double bytes;              /* algorithmic bytes (input + output) of those requests */
+  double span_s;             /* first arrival to last completion */
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* request latency */
+  long long batches;         /* dispatches (BATCHED) or launches (PLAIN) */
+  long long queue_launches;  /* persistent-kernel launches */
+  long long queued_requests; /* requests served by the persistent kernel */
+  double arena_peak_bytes;   /* high-water mark of the arena */
+  double table_bytes;        /* resident plan tables */
+} flagfftxServerStats;
+
+flagfftResult flagfftxServerCreate(flagfftxServer *server, size_t hbm_budget_bytes, int max_inflight, int mode);
+/* Build (and keep) the plans for one transform kind ahead of traffic. */
+flagfftResult flagfftxServerPrepare(flagfftxServer server, flagfftType type, int nx, int direction);
+flagfftResult flagfftxServerSubmit(flagfftxServer server, flagfftType type, int nx, int batch, int direction,
+                                   const void *idata, long long *ticket);
+/* done = 1 when complete; then *odata is the output in the arena. */
+flagfftResult flagfftxServerPoll(flagfftxServer server, long long ticket, int *done, void **odata,
+                                 double *latency_ms);
+flagfftResult flagfftxServerWait(flagfftxServer server, long long ticket, void **odata, double *latency_ms);
+/* Return a completed request's output memory to the arena. */
+flagfftResult flagfftxServerRelease(flagfftxServer server, long long ticket);
+flagfftResult flagfftxServerGetStats(flagfftxServer server, flagfftxServerStats *stats);
+flagfftResult flagfftxServerResetStats(flagfftxServer server);
+/* Waits for every submitted request, then frees the server. */
+flagfftResult flagfftxServerDestroy(flagfftxServer server);
+
 #ifdef __cplusplus
 }
 #endif

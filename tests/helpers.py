@@ -103,3 +103,30 @@ def run_c2c(dev, x: np.ndarray, direction: int, n: int, batch: int, inplace: boo
         (p.z2z if dbl else p.c2c)(xd.data_ptr(), yd.data_ptr(), direction)
         sync(dev)
     return host(yd)
+
+
+_HIP = None
+
+
+def hip_runtime():
+    """The HIP runtime library this process already loaded (torch's and the library's are one)."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        import os
+        import re
+
+        paths = sorted(set(re.findall(r"\S*libamdhip64\.so\S*", open("/proc/self/maps").read())))
+        assert paths, "HIP runtime not loaded"
+        _HIP = ctypes.CDLL(paths[0], mode=os.RTLD_NOLOAD | os.RTLD_GLOBAL)
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _HIP.hipMemcpy.restype = ctypes.c_int
+    return _HIP
+
+
+def d2h(ptr: int, shape, dtype) -> np.ndarray:
+    """Copy a raw device allocation into a new host array (synchronous)."""
+    out = np.empty(shape, dtype=dtype)
+    rc = hip_runtime().hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+    assert rc == 0, f"hipMemcpy failed ({rc})"
+    return out

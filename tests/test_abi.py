@@ -110,6 +110,28 @@ def test_compute_mode_validation():
             assert r == R.INVALID_DEVICE and not h.value
 
 
+def test_server_validation():
+    """flagfftxServer*: argument checks before any device work; null handles."""
+    import ctypes
+
+    import torch
+
+    L = ff.lib()
+    h = ctypes.c_void_p()
+    assert L.flagfftxServerCreate(None, 1 << 20, 2, 0) == R.INVALID_VALUE
+    assert L.flagfftxServerCreate(ctypes.byref(h), 0, 2, 0) == R.INVALID_VALUE
+    assert L.flagfftxServerCreate(ctypes.byref(h), 1 << 20, 0, 0) == R.INVALID_VALUE
+    assert L.flagfftxServerCreate(ctypes.byref(h), 1 << 20, 2, 7) == R.INVALID_VALUE
+    if not torch.cuda.is_available():
+        assert L.flagfftxServerCreate(ctypes.byref(h), 1 << 20, 2, 0) == R.INVALID_DEVICE and not h.value
+    t = ctypes.c_longlong()
+    assert L.flagfftxServerSubmit(None, int(T.C2C), 64, 1, -1, 8, ctypes.byref(t)) == R.INVALID_PLAN
+    assert L.flagfftxServerWait(None, 1, None, None) == R.INVALID_PLAN
+    assert L.flagfftxServerRelease(None, 1) == R.INVALID_PLAN
+    assert L.flagfftxServerGetStats(None, None) == R.INVALID_PLAN
+    assert L.flagfftxServerDestroy(None) == R.INVALID_PLAN
+
+
 def test_null_handles():
     L = ff.lib()
     assert L.flagfftDestroy(None) == R.INVALID_PLAN

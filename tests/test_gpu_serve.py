@@ -1,0 +1,144 @@
+"""Serving loop (flagfftxServer*): continuous batching, admission, alone == in the mix (GPU).
+
+Every request's output is compared three ways:
+  * bit-identical between the request served alone and served inside a mix
+    of other lengths, types and precisions (same kernels, same arguments);
+  * against the FP64 oracle within the stated bounds (FP32 2e-6 log2 N,
+    FP64 1e-14 log2 N, relative L2 per transform);
+  * PLAIN mode (per-request plan launches) against the same oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flagfft_amd import flagfft as ff
+from oracle import oracle as ora
+
+from .helpers import complex_input, d2h, dev_tensor, real_input, sync
+
+pytestmark = pytest.mark.gpu
+
+T = ff.Type
+# (type, n, batch, direction): single-pass (queued) and multi-pass requests
+MIX = [
+    (T.C2C, 64, 5, ff.FORWARD), (T.C2C, 97, 3, ff.INVERSE), (T.C2C, 1000, 2, ff.FORWARD),
+    (T.C2C, 4096, 4, ff.FORWARD), (T.C2C, 1009, 2, ff.FORWARD), (T.C2C, 2 * 3 * 5 * 7 * 11, 3, ff.INVERSE),
+    (T.C2C, 20000, 2, ff.FORWARD), (T.C2C, 65536, 1, ff.FORWARD), (T.C2C, 10007, 2, ff.FORWARD),
+    (T.R2C, 6000, 3, 0), (T.R2C, 256, 7, 0), (T.R2C, 999, 2, 0), (T.C2R, 512, 4, 0), (T.C2R, 6000, 2, 0),
+    (T.Z2Z, 128, 3, ff.FORWARD), (T.Z2Z, 1009, 2, ff.INVERSE), (T.Z2Z, 10935, 1, ff.FORWARD),
+    (T.Z2Z, 3000, 2, ff.FORWARD), (T.D2Z, 1000, 2, 0), (T.Z2D, 2048, 2, 0),
+]
+DBL = {T.Z2Z, T.D2Z, T.Z2D}
+
+
+def _inputs(dev):
+    xs, hosts = [], []
+    for i, (t, n, b, _) in enumerate(MIX):
+        dbl = t in DBL
+        if t in (T.C2C, T.Z2Z):
+            x = complex_input(500 + i, (b, n), np.complex128 if dbl else np.complex64, "normal")
+        elif t in (T.R2C, T.D2Z):
+            x = real_input(500 + i, (b, n), np.float64 if dbl else np.float32)
+        else:
+            full = ora.rfft(real_input(500 + i, (b, n), np.float64))
+            x = full.astype(np.complex128 if dbl else np.complex64)
+        hosts.append(x)
+        xs.append(dev_tensor(x, dev))
+    sync(dev)
+    return xs, hosts
+
+
+def _out_spec(t, n, b):
+    dbl = t in DBL
+    if t in (T.C2C, T.Z2Z):
+        return (b, n), np.complex128 if dbl else np.complex64
+    if t in (T.R2C, T.D2Z):
+        return (b, n // 2 + 1), np.complex128 if dbl else np.complex64
+    return (b, n), np.float64 if dbl else np.float32
+
+
+def _oracle(t, n, x, d):
+    x = np.asarray(x)
+    if t in (T.C2C, T.Z2Z):
+        return ora.fft(x.astype(np.complex128), sign=d)
+    if t in (T.R2C, T.D2Z):
+        return ora.rfft(x.astype(np.float64))
+    return ora.irfft(x.astype(np.complex128), n)
+
+
+def _bound(t, n):
+    return (1e-14 if t in DBL else 2e-6) * max(math.log2(n), 1.0)
+
+
+def _collect(srv, tickets):
+    outs = []
+    for tk, (t, n, b, _) in zip(tickets, MIX):
+        ptr, lat = srv.wait(tk)
+        assert lat >= 0
+        shape, dt = _out_spec(t, n, b)
+        outs.append(d2h(ptr, shape, dt))
+        srv.release(tk)
+    return outs
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(gpu):
+    xs, hosts = _inputs(gpu)
+    with ff.Server(1 << 30, max_inflight=3) as srv:
+        alone = []
+        for x, (t, n, b, d) in zip(xs, MIX):
+            tk = srv.submit(t, n, b, x.data_ptr(), d)
+            ptr, _ = srv.wait(tk)
+            shape, dt = _out_spec(t, n, b)
+            alone.append(d2h(ptr, shape, dt))
+            srv.release(tk)
+        order = np.random.default_rng(3).permutation(len(MIX))
+        tickets = [None] * len(MIX)
+        for i in order:
+            t, n, b, d = MIX[i]
+            tickets[i] = srv.submit(t, n, b, xs[i].data_ptr(), d)
+        mixed = _collect(srv, tickets)
+        st = srv.stats()
+    assert st["queue_launches"] >= 1 and st["queued_requests"] >= 10, st
+    for (t, n, b, d), a, m, x in zip(MIX, alone, mixed, hosts):
+        assert np.array_equal(a, m), f"{t.name} n={n}: result differs between alone and in the mix"
+        l2, _ = ora.rel_l2(m, _oracle(t, n, x, d), m.shape[-1])
+        assert l2 <= _bound(t, n), f"{t.name} n={n} rel_l2={l2:.3e}"
+
+
+def test_admission_by_hbm_budget(gpu):
+    """A budget that holds only a few requests at once still serves all of them, in bounds."""
+    xs, hosts = _inputs(gpu)
+    need = max(np.prod(_out_spec(t, n, b)[0]) * np.dtype(_out_spec(t, n, b)[1]).itemsize for t, n, b, _ in MIX)
+    budget = int(need * 6)  # outputs and workspaces of a handful of requests
+    with ff.Server(budget, max_inflight=2) as srv:
+        tickets = [srv.submit(t, n, b, x.data_ptr(), d) for x, (t, n, b, d) in zip(xs, MIX)]
+        outs = _collect(srv, tickets)
+        st = srv.stats()
+    assert st["completed"] == len(MIX)
+    assert st["arena_peak_bytes"] <= budget
+    for (t, n, b, d), y, x in zip(MIX, outs, hosts):
+        l2, _ = ora.rel_l2(y, _oracle(t, n, x, d), y.shape[-1])
+        assert l2 <= _bound(t, n), f"{t.name} n={n} rel_l2={l2:.3e}"
+
+
+def test_plain_mode_matches_the_oracle(gpu):
+    xs, hosts = _inputs(gpu)
+    with ff.Server(1 << 30, mode=ff.SERVE_PLAIN) as srv:
+        tickets = [srv.submit(t, n, b, x.data_ptr(), d) for x, (t, n, b, d) in zip(xs, MIX)]
+        outs = _collect(srv, tickets)
+        st = srv.stats()
+    assert st["queue_launches"] == 0 and st["completed"] == len(MIX)
+    assert st["transforms"] == sum(b for _, _, b, _ in MIX)
+    assert 0 <= st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    for (t, n, b, d), y, x in zip(MIX, outs, hosts):
+        l2, _ = ora.rel_l2(y, _oracle(t, n, x, d), y.shape[-1])
+        assert l2 <= _bound(t, n), f"{t.name} n={n} rel_l2={l2:.3e}"
+
+
+def test_request_larger_than_the_budget_is_refused(gpu):
+    x = dev_tensor(np.zeros((4, 4096), np.complex64), gpu)
+    with ff.Server(64 << 10, max_inflight=1) as srv:
+        with pytest.raises(ff.FlagFFTError) as e:
+            srv.submit(T.C2C, 4096, 4, x.data_ptr(), ff.FORWARD)
+        assert e.value.code == ff.Result.ALLOC_FAILED
