@@ -20,6 +20,7 @@
 #include "flagfft.h"
 #include "flagfft_ext.h"
 #include "planner.hpp"
+#include "tunedb.hpp"
 
 struct flagfftPlan_t {
   void *impl = nullptr;
@@ -52,6 +53,58 @@ struct PlanImpl {
 std::mutex g_handles_mu;
 std::set<flagfftHandle> g_handles;
 thread_local PlanOptions t_options;
+thread_local bool t_explicit = false;  // flagfftxSetTuning overrides the tuned-plan database
+
+bool overrides_to_options(const TuneOverrides &t, PlanOptions &o) {
+  if (t.leaf_max > 0) {
+    o.leaf_max_f32 = t.leaf_max;
+    o.leaf_max_f64 = t.leaf_max;
+  }
+  if (t.pass_max > 0) {
+    o.pass_max_f32 = t.pass_max;
+    o.pass_max_f64 = t.pass_max;
+  }
+  if (t.units > 0) o.forced_units = t.units;
+  if (t.ept == 8 || t.ept == 16 || t.ept == 32)
+    o.ept = t.ept;
+  else if (t.ept != 0)
+    return false;
+  if (t.chunk_bytes > 0) o.chunk_bytes = t.chunk_bytes;
+  if (t.passes.size() > 4 || t.radices.size() > 16) return false;
+  o.forced_passes = t.passes;
+  o.forced_radices = t.radices;
+  return true;
+}
+
+TuneKey tune_key(flagfftType type, int64_t n, int64_t batch) {
+  TuneKey k;
+  k.n = n;
+  k.batch = batch;
+  switch (type) {
+    case FLAGFFT_C2C: k.dtype = "complex64"; k.layout = "contiguous"; k.direction = "forward"; break;
+    case FLAGFFT_Z2Z: k.dtype = "complex128"; k.layout = "contiguous"; k.direction = "forward"; break;
+    case FLAGFFT_R2C: k.dtype = "float32"; k.layout = "real"; k.direction = "forward"; break;
+    case FLAGFFT_D2Z: k.dtype = "float64"; k.layout = "real"; k.direction = "forward"; break;
+    case FLAGFFT_C2R: k.dtype = "complex64"; k.layout = "hermitian"; k.direction = "inverse"; break;
+    case FLAGFFT_Z2D: k.dtype = "complex128"; k.layout = "hermitian"; k.direction = "inverse"; break;
+  }
+  return k;
+}
+
+// Options for a new plan: the thread's explicit tuning, else the newest tuned
+// row of the database (1D plans; C2C/Z2Z use the forward row for both
+// directions), else the defaults.  `source` names the database row used.
+PlanOptions plan_options(const Desc &d, std::string &source) {
+  source.clear();
+  if (t_explicit || d.rank != 1 || d.compute != 0) return t_options;
+  std::string js;
+  if (!tuned_lookup(tune_key(d.type, d.n[0], d.batch), js)) return t_options;
+  TuneOverrides t;
+  PlanOptions o;
+  if (!tuning_from_json(js, t) || !overrides_to_options(t, o)) return t_options;
+  source = tuned_db_path() + " " + js;
+  return o;
+}
 
 flagfftResult type_metadata(flagfftType type, Desc &d) {
   switch (type) {
@@ -153,17 +206,20 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
     auto impl = std::make_unique<PlanImpl>();
     impl->desc = d;
     impl->device = dev;
-    impl->tree = plan_tree(plan_desc(d), t_options);
+    std::string tuned;
+    const PlanOptions opts = plan_options(d, tuned);
+    impl->tree = plan_tree(plan_desc(d), opts);
     if (d.real_fwd) {
-      impl->fwd = build_executable(impl->tree, false, t_options, d.idist, d.odist);
+      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist);
     } else if (d.real_inv) {
-      impl->inv = build_executable(impl->tree, true, t_options, d.idist, d.odist);
+      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist);
     } else {
-      impl->fwd = build_executable(impl->tree, false, t_options, d.idist, d.odist);
-      impl->inv = build_executable(impl->tree, true, t_options, d.idist, d.odist);
+      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist);
+      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist);
     }
     std::ostringstream o;
     o << impl->tree.describe();
+    if (!tuned.empty()) o << "tuned-plan: " << tuned << "\n";
     o << "\n-- Forward execution --\n" << (impl->fwd ? impl->fwd->describe() : std::string("(not compiled)\n"));
     if (impl->inv) o << "\n-- Inverse execution --\n" << impl->inv->describe();
     impl->description = o.str();
@@ -347,8 +403,11 @@ extern "C" flagfftResult flagfftxPlanDescribe(int rank, const int *n, flagfftTyp
                                        static_cast<int>(out_len), type, rank == 2 ? 1 : batch, d);
   if (r != FLAGFFT_SUCCESS) return r;
   try {
-    flagfft::PlanTree t = flagfft::plan_tree(flagfft::plan_desc(d), flagfft::t_options);
+    std::string tuned;
+    const flagfft::PlanOptions opts = flagfft::plan_options(d, tuned);
+    flagfft::PlanTree t = flagfft::plan_tree(flagfft::plan_desc(d), opts);
     std::string s = t.describe();
+    if (!tuned.empty()) s += "tuned-plan: " + tuned + "\n";
     std::strncpy(buf, s.c_str(), len - 1);
     buf[len - 1] = '\0';
     return FLAGFFT_SUCCESS;
@@ -428,25 +487,55 @@ extern "C" flagfftResult flagfftxSetGraphMode(flagfftHandle handle, int enable) 
   return FLAGFFT_SUCCESS;
 }
 
+namespace flagfft {
+namespace {
+bool tuning_from_c(const flagfftxTuning *t, TuneOverrides &o) {
+  if (t->npasses < 0 || t->npasses > 4 || t->nradices < 0 || t->nradices > 16) return false;
+  o.leaf_max = t->leaf_max;
+  o.pass_max = t->pass_max;
+  o.units = t->units;
+  o.ept = t->ept;
+  o.chunk_bytes = t->chunk_bytes;
+  o.passes.assign(t->passes, t->passes + t->npasses);
+  o.radices.assign(t->radices, t->radices + t->nradices);
+  return true;
+}
+}  // namespace
+}  // namespace flagfft
+
 extern "C" flagfftResult flagfftxSetTuning(const flagfftxTuning *t) {
   flagfft::PlanOptions o;
   if (t != nullptr) {
-    if (t->leaf_max > 0) {
-      o.leaf_max_f32 = t->leaf_max;
-      o.leaf_max_f64 = t->leaf_max;
-    }
-    if (t->pass_max > 0) {
-      o.pass_max_f32 = t->pass_max;
-      o.pass_max_f64 = t->pass_max;
-    }
-    if (t->units > 0) o.forced_units = t->units;
-    if (t->ept == 8 || t->ept == 16 || t->ept == 32) o.ept = t->ept;
-    else if (t->ept != 0) return FLAGFFT_INVALID_VALUE;
-    if (t->chunk_bytes > 0) o.chunk_bytes = t->chunk_bytes;
-    if (t->npasses < 0 || t->npasses > 4 || t->nradices < 0 || t->nradices > 16) return FLAGFFT_INVALID_VALUE;
-    for (int i = 0; i < t->npasses; ++i) o.forced_passes.push_back(t->passes[i]);
-    for (int i = 0; i < t->nradices; ++i) o.forced_radices.push_back(t->radices[i]);
+    flagfft::TuneOverrides ov;
+    if (!flagfft::tuning_from_c(t, ov) || !flagfft::overrides_to_options(ov, o)) return FLAGFFT_INVALID_VALUE;
   }
   flagfft::t_options = o;
+  flagfft::t_explicit = t != nullptr;
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" flagfftResult flagfftxTunedStore(int nx, flagfftType type, int batch, const flagfftxTuning *t,
+                                            double median_us, double gbps, double default_us) {
+  if (t == nullptr || batch <= 0) return FLAGFFT_INVALID_VALUE;
+  if (nx <= 0) return FLAGFFT_INVALID_SIZE;
+  flagfft::Desc d;
+  if (flagfft::type_metadata(type, d) != FLAGFFT_SUCCESS) return FLAGFFT_INVALID_TYPE;
+  flagfft::TuneOverrides ov;
+  flagfft::PlanOptions o;
+  if (!flagfft::tuning_from_c(t, ov) || !flagfft::overrides_to_options(ov, o)) return FLAGFFT_INVALID_VALUE;
+  std::string err;
+  if (!flagfft::tuned_store(flagfft::tune_key(type, nx, batch), flagfft::tuning_to_json(ov), median_us, gbps,
+                            default_us, err)) {
+    std::fprintf(stderr, "[flagfft] tuned-plan store failed: %s\n", err.c_str());
+    return FLAGFFT_SETUP_FAILED;
+  }
+  return FLAGFFT_SUCCESS;
+}
+
+extern "C" flagfftResult flagfftxTunedDbPath(char *buf, size_t len) {
+  if (buf == nullptr || len == 0) return FLAGFFT_INVALID_VALUE;
+  const std::string p = flagfft::tuned_db_path();
+  std::strncpy(buf, p.c_str(), len - 1);
+  buf[len - 1] = '\0';
   return FLAGFFT_SUCCESS;
 }

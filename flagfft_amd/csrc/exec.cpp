@@ -181,8 +181,8 @@ void Executable::resolve_kernels() {
   for (Step &s : steps) {
     switch (s.kind) {
       case Step::Pass: s.func = pass_kernel(s.dbl, s.ept); break;
-      case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id); break;
-      case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id); break;
+      case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
+      case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
@@ -531,13 +531,14 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
     st.leaf_id = leaf_id;
     LeafArgs &a = st.la;
     const int R = static_cast<int>(c.n);
-    const int ept = leaf_kernel_ept(leaf_id);
+    const int ept = leaf_kernel_ept(leaf_id, dbl());
     const int tpu = (R + ept - 1) / ept;
     const int LS = odd_stride(lds_pad(R - 1) + 1);
     a.nseq = s.nbatch;
     a.idist = s.gi.bs;
     a.odist = s.go.bs;
     a.tw = roots(R, R, 1);
+    a.aux = nullptr;
     a.conj_in = m.conj_in;
     a.conj_out = m.conj_out;
     a.scale = m.scale;
@@ -553,7 +554,7 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
                            static_cast<size_t>(C) * LS * eb()};
     st.bytes_moved = s.nbatch * R * eb() * 2;
     std::ostringstream lab;
-    lab << "k_leaf<" << (dbl() ? "fp64" : "fp32") << "," << ept << ">[";
+    lab << "k_leaf<" << (dbl() ? "fp64" : "fp32") << "," << ept << ",0>[";
     for (size_t i = 0; i < c.radices.size(); ++i) lab << (i ? "x" : "") << c.radices[i];
     lab << "] R=" << R << " C=" << C << " tpu=" << tpu << " transforms=" << s.nbatch;
     st.label = lab.str();
@@ -620,6 +621,147 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
   ex_.steps.push_back(st);
 }
 
+PassArgs Builder::chain_pass(const std::vector<int> &radices, int R, int64_t p, int64_t n, const SeqSet &s,
+                             const Geometry &gi, const Geometry &go, const TwoLevel &tl) {
+  const int64_t tcnt = n / R;
+  PassArgs a{};
+  a.gi = gi;
+  a.go = go;
+  a.R = R;
+  a.tcnt = static_cast<int32_t>(tcnt);
+  a.p = static_cast<int32_t>(p);
+  a.nseq = n;
+  a.ncol = static_cast<int32_t>(s.ncol);
+  a.col_fast = s.col_fast ? 1 : 0;
+  a.units = s.nbatch * s.ncol * tcnt;
+  if (a.units >= (int64_t{1} << 31)) throw std::runtime_error("too many units for one launch");
+  fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, radices, R);
+  a.tw_small = roots(R, R, 1);
+  a.tw_lo = tl.lo;
+  a.tw_hi = tl.hi;
+  a.tw_bits = tl.bits;
+  a.twp_mult = static_cast<int32_t>(tcnt / p);
+  a.src_mode = kSrcComplex;
+  a.dst_mode = kDstComplex;
+  a.conj_in = a.conj_out = 0;
+  a.scale = 1.0;
+  a.nlog = n;
+  a.chirp = nullptr;
+  a.bhat = nullptr;
+  return a;
+}
+
+Builder::TwoLevel Builder::two_level(int64_t n) {
+  TwoLevel t;
+  t.bits = (ceil_log2(n) + 1) / 2;
+  const int64_t lo_count = int64_t{1} << t.bits;
+  const int64_t hi_count = (n + lo_count - 1) / lo_count;
+  t.lo = roots(lo_count, n, 1);
+  t.hi = roots(hi_count, n, lo_count);
+  return t;
+}
+
+bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst,
+                        const SeqSet &s, int64_t n, int pass, int npasses) {
+  const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices);
+  if (sid < 0 || s.col_fast || s.ncol != 1 || a.gi.es != 1 || a.go.es != 1 || (a.src_mode & 3) != kSrcComplex ||
+      (a.dst_mode & 3) != kDstComplex)
+    return false;
+  const int R = a.R;
+  const int64_t tcnt = a.tcnt, p = a.p;
+  int cu = 0, ept = 0;
+  stage_kernel_shape(sid, dbl(), cu, ept);
+  const int tpu = (R + ept - 1) / ept;
+  const int LS = odd_stride(lds_pad(R - 1) + 1);
+  Step sst;
+  sst.kind = Step::Stage;
+  sst.dbl = dbl();
+  sst.src = src;
+  sst.dst = dst;
+  sst.leaf_id = sid;
+  sst.leaf_mode = mode;
+  StageArgs &g = sst.sa;
+  g.bs_in = a.gi.bs;
+  g.bs_out = a.go.bs;
+  g.tcnt = static_cast<int32_t>(tcnt);
+  g.p = static_cast<int32_t>(p);
+  g.twp_mult = a.twp_mult;
+  g.tw_bits = a.tw_bits;
+  g.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
+  g.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+  g.tw_small = a.tw_small;
+  g.tw_lo = a.tw_lo;
+  g.tw_hi = a.tw_hi;
+  g.conj_in = a.conj_in;
+  g.conj_out = a.conj_out;
+  g.scale = a.scale;
+  g.units = a.units;
+  g.block_store = (mode == 1 || (p == 1 && a.dst_mode == kDstComplex && a.go.bs == n)) ? 1 : 0;
+  if (mode == 1 && a.go.bs != n) throw std::runtime_error("convolution pass needs contiguous sequences");
+  g.src_mode = a.src_mode;
+  g.dst_mode = a.dst_mode;
+  g.nlog = a.nlog;
+  g.chirp = a.chirp;
+  g.bhat = a.bhat;
+  const int64_t nblocks = (a.units + cu - 1) / cu;
+  g.nblocks = static_cast<int32_t>(nblocks);
+  g.xcd_remap = opt_.xcd_remap;
+  sst.ept = ept;
+  sst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cu * tpu)),
+                          static_cast<size_t>(cu) * LS * eb()};
+  sst.bytes_moved = s.nbatch * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+  std::ostringstream lab;
+  lab << "k_stage<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << cu << "," << mode << ">[";
+  for (size_t ri = 0; ri < radices.size(); ++ri) lab << (ri ? "x" : "") << radices[ri];
+  lab << "] " << (mode == 1 ? "convolution" : "pass") << " " << pass << "/" << npasses << " R=" << R << " p=" << p
+      << " tcnt=" << tcnt << (g.block_store ? " block-store" : "");
+  sst.label = lab.str();
+  ex_.steps.push_back(sst);
+  return true;
+}
+
+void Builder::emit_pass_step(const PassArgs &a0, int src, int dst, const SeqSet &s, int64_t n, int pass,
+                             int npasses) {
+  Step st;
+  st.kind = Step::Pass;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  PassArgs a = a0;
+  const int R = a.R;
+  const int64_t tcnt = a.tcnt, p = a.p;
+  const int LS = odd_stride(lds_pad(R - 1) + 1);
+  const int64_t in_i_stride = a.gi.es;
+  const int64_t out_i_stride = (p > 1) ? a.go.es : static_cast<int64_t>(R) * a.go.es;
+  const bool lsf = slot_fast(tcnt * a.gi.es, unit_stride(s.col_fast, tcnt, s.ncol, a.gi, in_i_stride));
+  const bool ssf = slot_fast(p * a.go.es, unit_stride(s.col_fast, tcnt, s.ncol, a.go, out_i_stride));
+  ShapeChoice sh = pick_shape(opt_, eb(), R, LS, true, a.units);
+  a.C = sh.C;
+  a.tpu = sh.tpu;
+  a.lds_stride = LS;
+  a.load_slot_fast = lsf;
+  a.store_slot_fast = ssf;
+  a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
+  a.fd_R = make_fastdiv(static_cast<uint32_t>(R));
+  a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  a.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
+  a.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+  const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
+  a.nblocks = static_cast<int32_t>(nblocks);
+  a.xcd_remap = opt_.xcd_remap;
+  a.direct = 0;
+  st.pa = a;
+  st.ept = sh.ept;
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
+  st.bytes_moved = s.nbatch * s.ncol *
+                   (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+  std::ostringstream lab;
+  lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> stockham-pass " << pass << "/" << npasses
+      << " R=" << R << " p=" << p << " tcnt=" << tcnt << " C=" << sh.C << " tpu=" << sh.tpu;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+}
+
 void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, const Modes &first, const Modes &last,
                              std::initializer_list<int> avoid_extra) {
   const size_t k = c.passes.size();
@@ -644,37 +786,13 @@ void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, c
     av2.push_back(wA);
     wB = pick(av2);
   }
-  const int bits = (ceil_log2(n) + 1) / 2;
-  const int64_t lo_count = int64_t{1} << bits;
-  const int64_t hi_count = (n + lo_count - 1) / lo_count;
-  const void *tw_lo = roots(lo_count, n, 1);
-  const void *tw_hi = roots(hi_count, n, lo_count);
+  const TwoLevel tl = two_level(n);
   int64_t p = 1;
   for (size_t j = 0; j < k; ++j) {
     const int R = static_cast<int>(c.passes[j]);
-    const int64_t tcnt = n / R;
-    Step st;
-    st.kind = Step::Pass;
-    st.dbl = dbl();
-    st.src = (j == 0) ? src : ((j % 2 == 1) ? wA : wB);
-    st.dst = (j + 1 == k) ? dst : (((j + 1) % 2 == 1) ? wA : wB);
-    PassArgs &a = st.pa;
-    a.gi = (j == 0) ? s.gi : gws;
-    a.go = (j + 1 == k) ? s.go : gws;
-    a.R = R;
-    a.tcnt = static_cast<int32_t>(tcnt);
-    a.p = static_cast<int32_t>(p);
-    a.nseq = n;
-    a.ncol = static_cast<int32_t>(s.ncol);
-    a.col_fast = s.col_fast ? 1 : 0;
-    a.units = s.nbatch * s.ncol * tcnt;
-    if (a.units >= (int64_t{1} << 31)) throw std::runtime_error("too many units for one launch");
-    fill_stages(a.rad, a.fd_q, a.twq_step, a.nst, c.pass_radices[j], R);
-    a.tw_small = roots(R, R, 1);
-    a.tw_lo = tw_lo;
-    a.tw_hi = tw_hi;
-    a.tw_bits = bits;
-    a.twp_mult = static_cast<int32_t>(tcnt / p);
+    const int st_src = (j == 0) ? src : ((j % 2 == 1) ? wA : wB);
+    const int st_dst = (j + 1 == k) ? dst : (((j + 1) % 2 == 1) ? wA : wB);
+    PassArgs a = chain_pass(c.pass_radices[j], R, p, n, s, (j == 0) ? s.gi : gws, (j + 1 == k) ? s.go : gws, tl);
     a.src_mode = (j == 0) ? first.src_mode : kSrcComplex;
     a.conj_in = (j == 0) ? first.conj_in : 0;
     a.dst_mode = (j + 1 == k) ? last.dst_mode : kDstComplex;
@@ -682,90 +800,10 @@ void Builder::emit_multipass(const Core &c, const SeqSet &s, int src, int dst, c
     a.scale = (j + 1 == k) ? last.scale : 1.0;
     const int64_t nl = (j == 0) ? first.nlog : last.nlog;
     a.nlog = nl ? nl : n;
-    a.chirp = nullptr;
-    a.bhat = nullptr;  // the pointwise pass of a Bluestein chain gets its table from emit_bluestein
     if ((a.src_mode & kSrcChirpFlag) || (a.dst_mode & kDstChirpFlag)) a.chirp = chirp(a.nlog);
-    const int LS = odd_stride(lds_pad(R - 1) + 1);
-    // compile-time specialised super-stage for contiguous complex sequences
-    const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(c.pass_radices[j]);
-    if (sid >= 0 && !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1 && (a.src_mode & 3) == kSrcComplex &&
-        (a.dst_mode & 3) == kDstComplex) {
-      int cu = 0, ept = 0;
-      stage_kernel_shape(sid, dbl(), cu, ept);
-      const int tpu = (R + ept - 1) / ept;
-      Step sst;
-      sst.kind = Step::Stage;
-      sst.dbl = dbl();
-      sst.src = st.src;
-      sst.dst = st.dst;
-      sst.leaf_id = sid;
-      StageArgs &g = sst.sa;
-      g.bs_in = a.gi.bs;
-      g.bs_out = a.go.bs;
-      g.tcnt = static_cast<int32_t>(tcnt);
-      g.p = static_cast<int32_t>(p);
-      g.twp_mult = a.twp_mult;
-      g.tw_bits = bits;
-      g.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
-      g.fd_p = make_fastdiv(static_cast<uint32_t>(p));
-      g.tw_small = a.tw_small;
-      g.tw_lo = tw_lo;
-      g.tw_hi = tw_hi;
-      g.conj_in = a.conj_in;
-      g.conj_out = a.conj_out;
-      g.scale = a.scale;
-      g.units = a.units;
-      g.block_store = (p == 1 && a.dst_mode == kDstComplex && a.go.bs == n) ? 1 : 0;
-      g.src_mode = a.src_mode;
-      g.dst_mode = a.dst_mode;
-      g.nlog = a.nlog;
-      g.chirp = a.chirp;
-      g.bhat = nullptr;
-      const int64_t nblocks = (a.units + cu - 1) / cu;
-      g.nblocks = static_cast<int32_t>(nblocks);
-      g.xcd_remap = opt_.xcd_remap;
-      sst.ept = ept;
-      sst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cu * tpu)),
-                              static_cast<size_t>(cu) * LS * eb()};
-      sst.bytes_moved = s.nbatch * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
-      std::ostringstream lab;
-      lab << "k_stage<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << cu << ">[";
-      for (size_t ri = 0; ri < c.pass_radices[j].size(); ++ri) lab << (ri ? "x" : "") << c.pass_radices[j][ri];
-      lab << "] pass " << j << "/" << k << " R=" << R << " p=" << p << " tcnt=" << tcnt
-          << (g.block_store ? " block-store" : "");
-      sst.label = lab.str();
-      ex_.steps.push_back(sst);
-      p *= R;
-      continue;
-    }
-    const int64_t in_i_stride = a.gi.es;
-    const int64_t out_i_stride = (p > 1) ? a.go.es : static_cast<int64_t>(R) * a.go.es;
-    const bool lsf = slot_fast(tcnt * a.gi.es, unit_stride(s.col_fast, tcnt, s.ncol, a.gi, in_i_stride));
-    const bool ssf = slot_fast(p * a.go.es, unit_stride(s.col_fast, tcnt, s.ncol, a.go, out_i_stride));
-    ShapeChoice sh = pick_shape(opt_, eb(), R, LS, true, a.units);
-    a.C = sh.C;
-    a.tpu = sh.tpu;
-    a.lds_stride = LS;
-    a.load_slot_fast = lsf;
-    a.store_slot_fast = ssf;
-    a.fd_C = make_fastdiv(static_cast<uint32_t>(sh.C));
-    a.fd_R = make_fastdiv(static_cast<uint32_t>(R));
-    a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
-    a.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
-    a.fd_p = make_fastdiv(static_cast<uint32_t>(p));
-    const int64_t nblocks = (a.units + sh.C - 1) / sh.C;
-    a.nblocks = static_cast<int32_t>(nblocks);
-    a.xcd_remap = opt_.xcd_remap;
-    a.direct = 0;
-    st.ept = sh.ept;
-    st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(sh.C * sh.tpu)), sh.lds};
-    st.bytes_moved = s.nbatch * s.ncol *
-                     (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
-    std::ostringstream lab;
-    lab << "k_pass<" << (dbl() ? "fp64" : "fp32") << "," << sh.ept << "> stockham-pass " << j << "/" << k << " R=" << R
-        << " p=" << p << " tcnt=" << tcnt << " C=" << sh.C << " tpu=" << sh.tpu;
-    st.label = lab.str();
-    ex_.steps.push_back(st);
+    // the pointwise pass of a Bluestein chain gets its table from emit_bluestein
+    if (!try_stage(a, c.pass_radices[j], 0, st_src, st_dst, s, n, static_cast<int>(j), static_cast<int>(k)))
+      emit_pass_step(a, st_src, st_dst, s, n, static_cast<int>(j), static_cast<int>(k));
     p *= R;
   }
 }
@@ -855,6 +893,42 @@ void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst,
 
 void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
   const int64_t n = c.n, M = c.m;
+  if (c.conv && c.conv->passes.size() == 2 && !opt_.generic_kernels && !s.col_fast && s.ncol == 1 &&
+      s.gi.es == 1 && s.go.es == 1 && (m.src_mode & 3) == kSrcComplex && (m.dst_mode & 3) == kDstComplex &&
+      stage_kernel_id(c.conv->pass_radices[0]) >= 0 && stage_kernel_id(c.conv->pass_radices[1]) >= 0) {
+    // Three launches over M = R0 * R1:
+    //   A  chirp * x, forward pass 0 (R0)                        user -> wA
+    //   B  forward pass 1 (R1), conj(X * bhat), inverse pass 0 (R1, the
+    //      inverse chain runs its super-radices in the opposite order)  wA -> wB
+    //   C  inverse pass 1 (R0), conj, * chirp, first n              wB -> user
+    const Core &cv = *c.conv;
+    const int R0 = static_cast<int>(cv.passes[0]), R1 = static_cast<int>(cv.passes[1]);
+    const Geometry gws{1, M, M};
+    const size_t wbytes = static_cast<size_t>(s.nbatch * M) * eb();
+    const int wA = pick_ws({src, dst}, wbytes);
+    const int wB = pick_ws({src, dst, wA}, wbytes);
+    const TwoLevel tl = two_level(M);
+    const void *ch = chirp(n);
+    const void *bh = bhat_bluestein(n, M);
+    SeqSet sw = s;
+    PassArgs a = chain_pass(cv.pass_radices[0], R0, 1, M, sw, s.gi, gws, tl);
+    a.src_mode = (m.src_mode & 3) | kSrcChirpFlag;
+    a.conj_in = m.conj_in;
+    a.nlog = n;
+    a.chirp = ch;
+    if (!try_stage(a, cv.pass_radices[0], 0, src, wA, sw, M, 0, 3)) throw std::runtime_error("bluestein pass A");
+    PassArgs b = chain_pass(cv.pass_radices[1], R1, R0, M, sw, gws, gws, tl);
+    b.bhat = bh;
+    if (!try_stage(b, cv.pass_radices[1], 1, wA, wB, sw, M, 1, 3)) throw std::runtime_error("bluestein pass B");
+    PassArgs cc = chain_pass(cv.pass_radices[0], R0, R1, M, sw, gws, s.go, tl);
+    cc.dst_mode = (m.dst_mode & 3) | kDstChirpFlag;
+    cc.conj_out = m.conj_out;
+    cc.scale = m.scale;
+    cc.nlog = n;
+    cc.chirp = ch;
+    if (!try_stage(cc, cv.pass_radices[0], 0, wB, dst, sw, M, 2, 3)) throw std::runtime_error("bluestein pass C");
+    return;
+  }
   if (c.conv) {
     // two multi-pass length-M transforms; chirp / pointwise / finalize fused into their end passes
     const Geometry gws = s.col_fast ? Geometry{s.ncol, 1, M * s.ncol} : Geometry{1, M, M * s.ncol};
@@ -972,6 +1046,50 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
   }
   const int64_t h = n / 2;
   if (pl.real_form == RealForm::EvenFused) {
+    const int lid = opt_.generic_kernels ? -1 : leaf_kernel_id(pl.core.radices);
+    if (lid >= 0 && s.ncol == 1 && !s.col_fast && s.gi.es == 1 && s.go.es == 1 && h > 1 &&
+        (pl.real_fwd ? s.gi.bs % 2 == 0 : s.go.bs % 2 == 0)) {
+      // specialised leaf with the split-spectrum pack / unpack fused
+      Step st;
+      st.kind = Step::Leaf;
+      st.leaf_mode = pl.real_fwd ? 1 : 2;
+      st.leaf_id = lid;
+      st.dbl = dbl();
+      st.src = src;
+      st.dst = dst;
+      LeafArgs &a = st.la;
+      const int H = static_cast<int>(h);
+      const int ept = leaf_kernel_ept(lid, dbl());
+      const int tpu = (H + ept - 1) / ept;
+      const int LS = odd_stride(lds_pad(H - 1) + 1);
+      a.nseq = s.nbatch;
+      a.idist = pl.real_fwd ? s.gi.bs / 2 : s.gi.bs;  // complex-pair units
+      a.odist = pl.real_fwd ? s.go.bs : s.go.bs / 2;
+      a.tw = roots(h, h, 1);
+      a.aux = roots(h + 1, n, 1);
+      a.conj_in = 0;
+      a.conj_out = pl.real_fwd ? 0 : 1;
+      a.scale = 1.0;
+      int C = std::max(1, opt_.target_threads / tpu);
+      if ((s.nbatch + C - 1) / C < 512) C = static_cast<int>(std::max<int64_t>(1, s.nbatch / 512));
+      if (opt_.forced_units > 0) C = opt_.forced_units;
+      while (C > 1 && (C * tpu > opt_.max_threads ||
+                       static_cast<size_t>(C) * LS * eb() > static_cast<size_t>(opt_.lds_pref_bytes)))
+        --C;
+      a.C = C;
+      const int64_t nblocks = (s.nbatch + C - 1) / C;
+      st.ept = ept;
+      st.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(C * tpu)),
+                             static_cast<size_t>(C) * LS * eb()};
+      st.bytes_moved = s.nbatch * (n * eb() / 2 + (h + 1) * eb());
+      std::ostringstream lab;
+      lab << "k_leaf<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << (pl.real_fwd ? 1 : 2) << ">[";
+      for (size_t i = 0; i < pl.core.radices.size(); ++i) lab << (i ? "x" : "") << pl.core.radices[i];
+      lab << "] " << (pl.real_fwd ? "R2C" : "C2R") << " n=" << n << " half=" << h << " C=" << C << " tpu=" << tpu;
+      st.label = lab.str();
+      ex_.steps.push_back(st);
+      return;
+    }
     Step st;
     st.kind = pl.real_fwd ? Step::R2CEven : Step::C2REven;
     st.dbl = dbl();

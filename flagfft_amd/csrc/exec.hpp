@@ -43,6 +43,7 @@ struct Step {
   bool dbl = false;
   int ept = 16;
   int leaf_id = -1;  // Leaf / Stage: entry of the specialised kernel table
+  int leaf_mode = 0;  // Leaf: 0 complex rows, 1 R2C, 2 C2R
   int src = kBufIn, dst = kBufOut;
   PassArgs pa{};
   LeafArgs la{};
@@ -154,6 +155,18 @@ class Builder {
   const void *fp64_transform_then_cast(const std::string &key, const std::shared_ptr<DevBuf> &src64, int64_t len,
                                        double scale);
 
+  struct TwoLevel {  // w_n^j = hi[j >> bits] * lo[j & (2^bits - 1)]
+    int bits = 0;
+    const void *lo = nullptr, *hi = nullptr;
+  };
+  TwoLevel two_level(int64_t n);
+  PassArgs chain_pass(const std::vector<int> &radices, int R, int64_t p, int64_t n, const SeqSet &s,
+                      const Geometry &gi, const Geometry &go, const TwoLevel &tl);
+  // Push a specialised super-stage step for `a` (mode 0 pass, 1 Bluestein
+  // convolution); false when no instance or the layout does not fit it.
+  bool try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst, const SeqSet &s,
+                 int64_t n, int pass, int npasses);
+  void emit_pass_step(const PassArgs &a, int src, int dst, const SeqSet &s, int64_t n, int pass, int npasses);
   void emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_multipass(const Core &c, const SeqSet &s, int src, int dst, const Modes &first, const Modes &last,
                       std::initializer_list<int> avoid_extra);

@@ -124,6 +124,7 @@ struct LeafArgs {
   int64_t nseq;          // transforms
   int64_t idist, odist;  // complex elements between consecutive transforms
   const void *tw;        // w_N^j, j < N
+  const void *aux;       // real rows: w_n^k, k <= N (n = 2N)
   int32_t conj_in, conj_out;
   double scale;
   int32_t C;             // transforms per workgroup

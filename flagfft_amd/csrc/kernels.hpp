@@ -34,14 +34,14 @@ LaunchShape real_pre_shape(const RealPassArgs &a);
 
 // Specialised single-pass kernels: id of the compiled radix sequence (-1 if none).
 int leaf_kernel_id(const std::vector<int> &radices);
-int leaf_kernel_ept(int id);
-const void *leaf_kernel(bool dbl, int id);
+int leaf_kernel_ept(int id, bool dbl);
+const void *leaf_kernel(bool dbl, int id, int mode);  // mode: 0 complex, 1 R2C, 2 C2R
 
 // Specialised super-stage kernels: id of the compiled radix sequence (-1 if none),
 // its units per workgroup and elements per thread.
 int stage_kernel_id(const std::vector<int> &radices);
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept);
-const void *stage_kernel(bool dbl, int id);
+const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 Bluestein convolution
 
 // Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
 // QueueArgs argument).

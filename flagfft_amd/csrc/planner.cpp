@@ -198,7 +198,14 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   c.n = n;
   std::vector<int> rad;
   const int64_t lmax = leaf_max(o, p);
-  if (n <= lmax && radix_factor(n, rad)) {
+  bool leaf = n <= lmax && radix_factor(n, rad);
+  if (!leaf && o.forced_passes.empty() && n <= 32768 && radix_factor(n, rad) && leaf_kernel_id(rad) >= 0 &&
+      static_cast<int64_t>(lds_pad(static_cast<int>(n) - 1) + 2) * elem_bytes(p) <= o.lds_cap_bytes) {
+    // a specialised instance holding the whole sequence in LDS: one HBM round
+    // trip instead of two (one workgroup per sequence)
+    leaf = true;
+  }
+  if (leaf) {
     c.kind = CoreKind::Leaf;
     if (!o.forced_radices.empty()) {
       int64_t prod = 1;

@@ -93,6 +93,8 @@ inline int lds_pad(int e) { return e + (e >> 5); }   // must match the kernels' 
 // Catalogue of the compile-time specialised super-stage kernels (leaf.hip):
 // id of the kernel for this radix sequence, -1 when none is compiled.
 int stage_kernel_id(const std::vector<int> &radices);
+// Same for the specialised single-pass kernels.
+int leaf_kernel_id(const std::vector<int> &radices);
 // True when every super-radix of the split has a specialised kernel.
 bool passes_specialised(const std::vector<int64_t> &passes);
 

@@ -55,7 +55,7 @@ EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "
                "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute",
                "flagfftxSetGraphMode", "flagfftxServerCreate", "flagfftxServerPrepare", "flagfftxServerSubmit",
                "flagfftxServerPoll", "flagfftxServerWait", "flagfftxServerRelease", "flagfftxServerGetStats",
-               "flagfftxServerResetStats", "flagfftxServerDestroy"]
+               "flagfftxServerResetStats", "flagfftxServerDestroy", "flagfftxTunedStore", "flagfftxTunedDbPath"]
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
@@ -126,6 +126,9 @@ def lib():
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
             "flagfftxPlan1dCompute": [_HP, _int, _int, _int, _int],
             "flagfftxSetGraphMode": [_H, _int],
+            "flagfftxTunedStore": [_int, _int, _int, ctypes.POINTER(Tuning), ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double],
+            "flagfftxTunedDbPath": [ctypes.c_char_p, ctypes.c_size_t],
             "flagfftxServerCreate": [_HP, ctypes.c_size_t, _int, _int],
             "flagfftxServerPrepare": [_H, _int, _int, _int],
             "flagfftxServerSubmit": [_H, _int, _int, _int, _int, _vp, ctypes.POINTER(ctypes.c_longlong)],
@@ -245,10 +248,8 @@ def plan_stats(h, direction: int = FORWARD):
     return r, launches.value, nbytes.value
 
 
-def set_tuning(**kw) -> int:
-    """Plan-time tunables for plans created afterwards on this thread; no args restores defaults."""
-    if not kw:
-        return lib().flagfftxSetTuning(None)
+def make_tuning(**kw) -> Tuning:
+    """flagfftxTuning from keyword overrides (leaf_max, pass_max, units, ept, chunk_bytes, passes, radices)."""
     t = Tuning()
     for k, v in kw.items():
         if k == "passes":
@@ -260,9 +261,28 @@ def set_tuning(**kw) -> int:
             for i, x in enumerate(v):
                 t.radices[i] = x
         else:
-            setattr(t, k, v)
+            setattr(t, k, int(v))
+    return t
+
+
+def set_tuning(**kw) -> int:
+    """Plan-time tunables for plans created afterwards on this thread; no args restores defaults."""
+    if not kw:
+        return lib().flagfftxSetTuning(None)
+    t = make_tuning(**kw)
     return lib().flagfftxSetTuning(ctypes.byref(t))
 
+
+def tuned_store(n: int, type_: int, batch: int, median_us: float, gbps: float, default_us: float, **kw) -> int:
+    """Append a tuned-plan row (the overrides in kw) to the tuned-plan database."""
+    t = make_tuning(**kw)
+    return lib().flagfftxTunedStore(n, int(type_), batch, ctypes.byref(t), median_us, gbps, default_us)
+
+
+def tuned_db_path() -> str:
+    buf = ctypes.create_string_buffer(4096)
+    check("flagfftxTunedDbPath", lib().flagfftxTunedDbPath(buf, len(buf)))
+    return buf.value.decode()
 
 def set_profiling(h, enable: bool) -> int:
     return lib().flagfftxSetProfiling(h, 1 if enable else 0)

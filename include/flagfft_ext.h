@@ -60,6 +60,20 @@ typedef struct flagfftxTuning_t {
 
 flagfftResult flagfftxSetTuning(const flagfftxTuning *tuning); /* NULL restores defaults */
 
+/* Tuned-plan database.  Plan creation (1D, vector arithmetic, no explicit
+ * flagfftxSetTuning on the thread) looks up the newest valid row of table
+ * tuned_measurements for (gfx950, nx, batch bucket, input dtype, direction,
+ * layout) and applies its overrides; C2C/Z2Z use the forward row for both
+ * directions.  The file is $FLAGFFT_TUNE_DB, else tuned_plans.sqlite beside
+ * libflagfft.so; FLAGFFT_TUNE_DISABLE=1 turns lookups off.  Same table, key
+ * columns and batch buckets as the reference's cache
+ * (src/utils/plan_json.cpp:170-207, src/exec/tuned_plan.cpp:21-58).
+ * Store appends a row (the autotuner's winner, with its measured median and
+ * the default plan's for comparison). */
+flagfftResult flagfftxTunedStore(int nx, flagfftType type, int batch, const flagfftxTuning *tuning,
+                                 double median_us, double gbps, double default_us);
+flagfftResult flagfftxTunedDbPath(char *buf, size_t len);
+
 /* Graph mode: each Exec replays the plan's kernel launches as one hipGraph
  * (recorded at the first Exec of each direction; later Execs patch only the
  * nodes that touch the caller's buffers).  Results are identical to the

@@ -210,3 +210,45 @@ def test_tuning_overrides_reach_the_planner():
     finally:
         ff.set_tuning()
     assert "Leaf(n=4096, radices=[16,16,16]" in _describe(4096)
+
+
+def test_tuned_plan_database_round_trip(tmp_path, monkeypatch):
+    """A stored row changes the planner's choice for its key only; DISABLE turns lookups off."""
+    import sqlite3
+
+    db = tmp_path / "tuned_plans.sqlite"
+    monkeypatch.setenv("FLAGFFT_TUNE_DB", str(db))
+    monkeypatch.delenv("FLAGFFT_TUNE_DISABLE", raising=False)
+    assert ff.tuned_db_path() == str(db)
+    ff.set_tuning()
+    default = _describe(1 << 16, T.C2C, 4)
+    assert "super_radices=[256,256]" in default and "tuned-plan" not in default
+    assert ff.tuned_store(1 << 16, T.C2C, 4, 10.0, 100.0, 12.0, passes=[1024, 64]) == R.SUCCESS
+    tuned = _describe(1 << 16, T.C2C, 4)
+    assert "super_radices=[1024,64]" in tuned and "tuned-plan:" in tuned
+    # same batch bucket (2-8) matches; other buckets, types and lengths do not
+    assert "super_radices=[1024,64]" in _describe(1 << 16, T.C2C, 8)
+    assert "super_radices=[256,256]" in _describe(1 << 16, T.C2C, 64)
+    assert "super_radices=[256,256]" in _describe(1 << 16, T.Z2Z, 4) or "tuned-plan" not in _describe(1 << 16, T.Z2Z, 4)
+    # newest row wins
+    assert ff.tuned_store(1 << 16, T.C2C, 4, 9.0, 110.0, 12.0, passes=[64, 1024]) == R.SUCCESS
+    assert "super_radices=[64,1024]" in _describe(1 << 16, T.C2C, 4)
+    # explicit thread tuning beats the database
+    ff.set_tuning(passes=[512, 128])
+    assert "super_radices=[512,128]" in _describe(1 << 16, T.C2C, 4)
+    ff.set_tuning()
+    monkeypatch.setenv("FLAGFFT_TUNE_DISABLE", "1")
+    assert "tuned-plan" not in _describe(1 << 16, T.C2C, 4)
+    monkeypatch.delenv("FLAGFFT_TUNE_DISABLE")
+    # the table has the reference's key columns
+    with sqlite3.connect(db) as c:
+        cols = [r[1] for r in c.execute("PRAGMA table_info(tuned_measurements)")]
+        rows = c.execute("SELECT fft_length, batch_bucket, dtype, direction, norm, input_layout, status "
+                         "FROM tuned_measurements").fetchall()
+    for col in ("plan_json", "schema_version", "status", "rank", "device_arch", "fft_length", "batch_bucket", "dtype",
+                "direction", "norm", "input_layout", "planner_fingerprint", "codegen_fingerprint",
+                "runtime_fingerprint", "measured_at"):
+        assert col in cols
+    assert rows[0] == (1 << 16, "2-8", "complex64", "forward", "backward", "contiguous", "valid")
+    # invalid overrides are refused before anything is written
+    assert ff.tuned_store(1 << 16, T.C2C, 4, 1.0, 1.0, 1.0, ept=7) == R.INVALID_VALUE
