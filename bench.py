@@ -11,8 +11,9 @@ the timed region reads HBM.
 value          algorithmic HBM GB/s of the whole step: sum over transforms of
                batch * (input bytes + output bytes), divided by wall time
 roofline       dominant kernel (by total device time): algorithmic bytes per
-               launch / mean launch duration, both from HIP events recorded on
-               the plan's stream around every launch inside the timed region
+               launch / mean launch duration from HIP events recorded on the
+               plan's stream around every launch of a second timed region of
+               the same K steps (the first, event-free region gives value)
 cpu_baseline   the oracle's fast FFT (oracle/oracle.c) on the host cores over
                a bounded sample of the same step
 
@@ -153,18 +154,15 @@ def kernel_symbol(label: str) -> str:
 
 
 def measure(cases, steps, warmup, torch, ff, stream):
+    """Two timed regions of the same K steps on one stream:
+    1. throughput: nothing but the launches between the synchronisations (-> value);
+    2. per-launch HIP events recorded on the stream around every launch (-> roofline)."""
     for c in cases:
         c.bind(stream.cuda_stream)
-        for p in c.plans:
-            ff.check("flagfftxSetProfiling", ff.set_profiling(p.handle, True))
     with torch.cuda.stream(stream):
         for s in range(warmup):
             for c in cases:
                 c.run(s)
-        torch.cuda.synchronize()
-        for c in cases:  # restart the per-launch totals: the timed region only
-            for p in c.plans:
-                ff.set_profiling(p.handle, True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(steps):
@@ -172,6 +170,14 @@ def measure(cases, steps, warmup, torch, ff, stream):
                 c.run(warmup + s)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        for c in cases:  # per-launch events from here on
+            for p in c.plans:
+                ff.check("flagfftxSetProfiling", ff.set_profiling(p.handle, True))
+        torch.cuda.synchronize()
+        for s in range(steps):
+            for c in cases:
+                c.run(warmup + steps + s)
+        torch.cuda.synchronize()
     launches = {}
     per_case = {}
     for c in cases:
@@ -233,6 +239,109 @@ def cpu_baseline(cases, budget_s=12.0):
                       f"(oracle fast FFT, FP32 for f32 cases, FP64 for f64 cases, {total_t:.1f} s wall)"}
 
 
+def _is_prime(n: int) -> bool:
+    if n < 2:
+        return False
+    d = 2
+    while d * d <= n:
+        if n % d == 0:
+            return False
+        d += 1
+    return True
+
+
+def serving_trace(nreq: int, seed: int, npool: int = 48):
+    """Requests of the serving configuration: length log-uniform in [64, 2^20] with
+    20% replaced by the nearest prime, batch uniform in [1, 512], type 70% C2C f32,
+    20% R2C f32, 10% Z2Z f64.  Lengths come from a pool of `npool` such draws so
+    that plans and tables are reused, as a serving system's would be."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(npool):
+        n = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1 << 20))))))
+        if rng.uniform() < 0.2:
+            while not _is_prime(n):
+                n += 1
+        pool.append(n)
+    reqs = []
+    for _ in range(nreq):
+        u = rng.uniform()
+        t = "C2C" if u < 0.7 else ("R2C" if u < 0.9 else "Z2Z")
+        n = pool[int(rng.integers(len(pool)))]
+        if t == "R2C" and n % 2:
+            n += 1  # real transforms of the pool's odd lengths use the next even length
+        reqs.append((t, n, int(rng.integers(1, 513))))
+    return reqs
+
+
+def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float = 48.0):
+    """Poisson stream through the server in BATCHED and PLAIN modes at the same offered rate."""
+    import threading
+
+    T = ff.Type
+    reqs = serving_trace(nreq, seed)
+    esz = {"C2C": 8, "R2C": 4, "Z2Z": 16}
+    need = {t: max([n * b * esz[t] for tt, n, b in reqs if tt == t] or [0]) for t in esz}
+    inputs = {t: torch.randn(max(need[t] // 4, 1), dtype=torch.float32, device=dev) for t in esz}
+    torch.cuda.synchronize()
+    tmap = {"C2C": T.C2C, "R2C": T.R2C, "Z2Z": T.Z2Z}
+    out = {}
+
+    def run(mode, rate):
+        with ff.Server(int(budget_gib * (1 << 30)), max_inflight=3, mode=mode) as srv:
+            for t, n, _ in set((t, n, 0) for t, n, b in reqs):
+                srv.prepare(tmap[t], n, ff.FORWARD)
+            tickets = []
+            done = threading.Event()
+
+            def reaper():
+                i = 0
+                while i < nreq:
+                    if i < len(tickets):
+                        srv.wait(tickets[i])
+                        srv.release(tickets[i])
+                        i += 1
+                    else:
+                        time.sleep(20e-6)
+                done.set()
+
+            th = threading.Thread(target=reaper, daemon=True)
+            th.start()
+            rng = np.random.default_rng(seed + 1)
+            gaps = rng.exponential(1.0 / rate, nreq) if rate else np.zeros(nreq)
+            t0 = time.perf_counter()
+            t_next = t0
+            for (t, n, b), g in zip(reqs, gaps):
+                t_next += g
+                while time.perf_counter() < t_next:
+                    pass
+                tickets.append(srv.submit(tmap[t], n, b, inputs[t].data_ptr(), ff.FORWARD))
+            done.wait()
+            th.join()
+            return srv.stats()
+
+    # capacity: every request submitted at once; the plain-launch capacity sets the offered rate
+    caps = {name: run(mode, 0.0) for name, mode in (("batched", ff.SERVE_BATCHED), ("plain", ff.SERVE_PLAIN))}
+    cap_rate = caps["plain"]["completed"] / caps["plain"]["span_s"]
+    rate = 0.8 * cap_rate
+    for name, mode in (("batched", ff.SERVE_BATCHED), ("plain", ff.SERVE_PLAIN)):
+        s = run(mode, rate)
+        out[name] = {"transforms_per_s": round(s["transforms"] / s["span_s"], 1),
+                     "requests_per_s": round(s["completed"] / s["span_s"], 1),
+                     "gbps": round(s["bytes"] / s["span_s"] / 1e9, 1),
+                     "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+                     "queue_launches": s["queue_launches"], "queued_requests": s["queued_requests"]}
+    out["capacity_transforms_per_s"] = {k: round(v["transforms"] / v["span_s"], 1) for k, v in caps.items()}
+    out["capacity_gbps"] = {k: round(v["bytes"] / v["span_s"] / 1e9, 1) for k, v in caps.items()}
+    out["offered_requests_per_s"] = round(rate, 1)
+    out["requests"] = nreq
+    out["plain_capacity_requests_per_s"] = round(cap_rate, 1)
+    out["trace"] = ("capacity: all requests at once; latency: Poisson arrivals at 80% of the plain-launch "
+                    "capacity; N log-uniform [64, 2^20] from a pool "
+                    "of 48 lengths (20% prime); batch U[1,512]; 70% C2C f32, 20% R2C f32, 10% Z2Z f64")
+    return out
+
+
 def run_workload(spec, args, torch, ff, dev, stream, seed):
     cases = [Case(name, kind, n, batch, dbl, dev, ff, torch) for name, kind, n, batch, dbl in spec]
     footprint = sum(c.footprint() for c in cases)
@@ -258,6 +367,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--suite", action="store_true", help="also measure the other configurations (stderr)")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -273,6 +383,13 @@ def main():
     cases, res = run_workload(WORKLOAD, args, torch, ff, dev, stream, 1234)
 
     cpu = None if args.no_cpu else cpu_baseline(cases)
+    transforms_per_step = sum(c.batch * (1 if c.kind == "c2c" else 2) for c in cases)
+    serving = None
+    if args.serve_requests > 0:
+        del cases
+        torch.cuda.empty_cache()
+        serving = serving_leg(torch, ff, dev, args.serve_requests)
+        cases = []
     line = {
         "metric": METRIC,
         "value": round(res["value"], 2),
@@ -289,7 +406,7 @@ def main():
         "config": {
             "workload": "arbitrary-length 1D: C2C f32 + Z2Z f64 forward at N=10007 (prime) and N=10935=3^7*5, "
                         "batch 64, N(0,1); R2C->C2R f32 + D2Z->Z2D f64 at N=6000, batch 128, U[-1,1]",
-            "transforms_per_step": sum(c.batch * (1 if c.kind == "c2c" else 2) for c in cases),
+            "transforms_per_step": transforms_per_step,
             "alg_bytes_per_step": res["alg_bytes_per_step"],
             "hbm_frac_of_peak": round(res["value"] / HBM_PEAK_GBS, 4),
             "pass_bytes_gbps": round(res["pass_gbps"], 2),
@@ -298,6 +415,7 @@ def main():
         },
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
+        "serving": serving,
         "per_case": {k: {"gbps": round(v["gbps"], 1) if v["gbps"] else None,
                          "device_us": round(v["device_ms_per_exec"] * 1e3, 2)} for k, v in res["per_case"].items()},
         "kernels": res["kernels"],

@@ -100,6 +100,11 @@ def candidates(type_, n: int, batch: int, describe: str):
         for s in splits[:8]:
             cands.append({"passes": s})
             cands.append({"passes": s[::-1]})
+        # wider tiles of the specialised super-stages (longer contiguous runs)
+        for units in (8, 16, 32, 64):
+            cands.append({"units": units})
+            for s in splits[:3]:
+                cands.append({"passes": s, "units": units})
     uniq, seen = [], set()
     for c in cands:
         key = repr(sorted(c.items()))

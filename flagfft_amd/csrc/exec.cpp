@@ -663,7 +663,7 @@ Builder::TwoLevel Builder::two_level(int64_t n) {
 
 bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst,
                         const SeqSet &s, int64_t n, int pass, int npasses) {
-  const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices);
+  const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
   if (sid < 0 || s.col_fast || s.ncol != 1 || a.gi.es != 1 || a.go.es != 1 || (a.src_mode & 3) != kSrcComplex ||
       (a.dst_mode & 3) != kDstComplex)
     return false;
@@ -1137,7 +1137,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     ex_.steps.push_back(st);
     return;
   }
-  // EvenMulti: n/2-point multi-pass chain plus a split-spectrum pass
+  // EvenMulti: n/2-point core (multi-pass chain, Rader or Bluestein) plus a split-spectrum pass
   const Geometry gws{1, h, h * s.ncol};
   const size_t wbytes = static_cast<size_t>(s.nbatch * s.ncol * h) * eb();
   const void *tw_half = roots(h + 1, n, 1);
@@ -1147,7 +1147,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     sc.gi = Geometry{1, s.gi.cs / 2, s.gi.bs / 2};
     sc.go = gws;
     Modes m;
-    emit_multipass(pl.core, sc, src, W, m, m, {dst});
+    emit_core(pl.core, sc, src, W, m);
     Step st;
     st.kind = Step::RealPost;
     st.dbl = dbl();
@@ -1182,9 +1182,8 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     sc.gi = gws;
     sc.go = Geometry{1, s.go.cs / 2, s.go.bs / 2};
     Modes m;
-    Modes last;
-    last.conj_out = true;
-    emit_multipass(pl.core, sc, W, dst, m, last, {src});
+    m.conj_out = true;
+    emit_core(pl.core, sc, W, dst, m);
   }
 }
 

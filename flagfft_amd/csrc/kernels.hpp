@@ -40,6 +40,8 @@ const void *leaf_kernel(bool dbl, int id, int mode);  // mode: 0 complex, 1 R2C,
 // Specialised super-stage kernels: id of the compiled radix sequence (-1 if none),
 // its units per workgroup and elements per thread.
 int stage_kernel_id(const std::vector<int> &radices);
+// The instance with `units` per workgroup (0: the default one), -1 if none.
+int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units);
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept);
 const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 Bluestein convolution
 

@@ -228,10 +228,21 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
     }
   }
   if (smooth13(n) && multipass_core(n, p, o, c)) return c;
-  // Bluestein: smallest 13-smooth convolution length M >= 2n-1 that plans; a
-  // multi-pass M up to 25% longer whose passes all have specialised kernels
-  // is preferred (those run 2-3x faster than the generic pass).
-  for (int64_t M = 2 * n - 1;; ++M) {
+  // Bluestein over a 13-smooth convolution length M >= 2n-1.  A length that
+  // fits one LDS pass is taken as soon as it is found (the whole convolution
+  // is one launch).  Otherwise every M up to twice the minimum is weighed by
+  // its HBM sweeps: M x passes, x 2.5 when a pass has no specialised kernel
+  // (the run-time-radix pass runs about that much slower).
+  Core best;
+  double best_cost = 0.0;
+  bool have = false;
+  const int64_t m0 = 2 * n - 1;
+  for (int64_t M = m0;; ++M) {
+    if (have && static_cast<double>(M) >= best_cost) break;
+    if (M > 2 * m0 + 64) {
+      if (have) break;
+      if (M > 64 * n + 1024) throw std::runtime_error("no Bluestein convolution length found");
+    }
     if (!smooth13(M)) continue;
     if (M <= lmax && radix_factor(M, rad)) {
       c.kind = CoreKind::Bluestein;
@@ -241,25 +252,25 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       return c;
     }
     Core sub;
-    if (multipass_core(M, p, o, sub)) {
-      if (o.forced_passes.empty() && !passes_specialised(sub.passes)) {
-        for (int64_t M2 = M + 1; M2 <= M + M / 4; ++M2) {
-          Core sub2;
-          if (smooth13(M2) && multipass_core(M2, p, o, sub2) && passes_specialised(sub2.passes)) {
-            M = M2;
-            sub = sub2;
-            break;
-          }
-        }
-      }
-      c.kind = CoreKind::Bluestein;
-      c.m = M;
-      c.radices.clear();
-      c.conv = std::make_shared<Core>(sub);
-      return c;
+    if (!multipass_core(M, p, o, sub)) continue;
+    if (!o.forced_passes.empty()) {
+      best = sub;
+      have = true;
+      break;
     }
-    if (M > 64 * n + 1024) throw std::runtime_error("no Bluestein convolution length found");
+    const double cost = static_cast<double>(M) * static_cast<double>(sub.passes.size()) / 2.0 *
+                        (passes_specialised(sub.passes) ? 1.0 : 2.5);
+    if (!have || cost < best_cost) {
+      best = sub;
+      best_cost = cost;
+      have = true;
+    }
   }
+  c.kind = CoreKind::Bluestein;
+  c.m = best.n;
+  c.radices.clear();
+  c.conv = std::make_shared<Core>(best);
+  return c;
 }
 
 std::string Core::describe(int indent) const {
@@ -320,17 +331,12 @@ Plan1D plan_1d(int64_t n, bool real_fwd, bool real_inv, Prec p, const PlanOption
     return pl;
   }
   if (n % 2 == 0 && n >= 2) {
+    // half-length complex core, pack / unpack fused (single pass) or as a
+    // split-spectrum pass beside the core (any other core kind)
     Core half = plan_core(n / 2, p, o);
-    if (half.kind == CoreKind::Leaf) {
-      pl.real_form = RealForm::EvenFused;
-      pl.core = half;
-      return pl;
-    }
-    if (half.kind == CoreKind::MultiPass) {
-      pl.real_form = RealForm::EvenMulti;
-      pl.core = half;
-      return pl;
-    }
+    pl.real_form = half.kind == CoreKind::Leaf ? RealForm::EvenFused : RealForm::EvenMulti;
+    pl.core = half;
+    return pl;
   }
   pl.real_form = RealForm::Full;
   pl.core = plan_core(n, p, o);

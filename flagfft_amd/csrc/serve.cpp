@@ -443,33 +443,49 @@ class Server {
         if (stop_) break;
       }
       if (stop_ && pending_.empty() && admitted_.empty() && inflight_ == 0) break;
-      Slot *slot = nullptr;
-      if (!admitted_.empty() && !failed_) {
-        for (Slot &s : slots_)
-          if (!s.busy || mode_ == FLAGFFTX_SERVE_PLAIN) {
-            slot = &s;
+      // Requests that run their own plan's kernels go, in arrival order, to
+      // lane 0 as soon as they are admitted; single-pass requests wait for a
+      // free batch slot (its pinned job staging must be idle) and leave
+      // together in one persistent-kernel launch.  With one slot, lane 0 is
+      // also the batch slot.
+      std::vector<Request *> direct, queued;
+      Slot *batch_slot = nullptr;
+      if (!failed_ && !admitted_.empty()) {
+        for (size_t si = (slots_.size() > 1 ? 1 : 0); si < slots_.size(); ++si)
+          if (!slots_[si].busy) {
+            batch_slot = &slots_[si];
             break;
           }
-      }
-      if (slot) {
-        std::vector<Request *> batch;
-        if (mode_ == FLAGFFTX_SERVE_PLAIN) {
-          batch.push_back(admitted_.front());
-          admitted_.pop_front();
-        } else {
-          while (!admitted_.empty() && batch.size() < static_cast<size_t>(kMaxJobs)) {
-            batch.push_back(admitted_.front());
-            admitted_.pop_front();
+        for (auto it = admitted_.begin(); it != admitted_.end();) {
+          Request *r = *it;
+          const bool q = r->queued && mode_ == FLAGFFTX_SERVE_BATCHED;
+          if (!q) {
+            direct.push_back(r);
+            it = admitted_.erase(it);
+          } else if (batch_slot != nullptr && queued.size() < static_cast<size_t>(kMaxJobs)) {
+            queued.push_back(r);
+            it = admitted_.erase(it);
+          } else {
+            ++it;
           }
         }
-        for (Request *r : batch) r->state = Request::InFlight;
-        inflight_ += static_cast<int64_t>(batch.size());
-        slot->busy = true;
+      }
+      if (!direct.empty() || !queued.empty()) {
+        for (Request *r : direct) r->state = Request::InFlight;
+        for (Request *r : queued) r->state = Request::InFlight;
+        inflight_ += static_cast<int64_t>(direct.size() + queued.size());
+        Slot &lane = slots_[0];
+        if (!direct.empty()) lane.busy = true;
+        if (!queued.empty()) batch_slot->busy = true;
         lk.unlock();
-        std::vector<Group> groups;
-        hipError_t e = launch(*slot, batch, groups);
+        std::vector<Group> lane_groups, batch_groups;
+        hipError_t e = hipSuccess;
+        if (!direct.empty()) e = launch(lane, direct, lane_groups);
+        if (e == hipSuccess && !queued.empty()) e = launch(*batch_slot, queued, batch_groups);
         lk.lock();
-        for (Group &g : groups) slot->groups.push_back(std::move(g));
+        for (Group &g : lane_groups) lane.groups.push_back(std::move(g));
+        if (!queued.empty())
+          for (Group &g : batch_groups) batch_slot->groups.push_back(std::move(g));
         if (e != hipSuccess) {
           std::fprintf(stderr, "[flagfft] server launch failed: %s\n", hipGetErrorString(e));
           failed_ = true;

@@ -241,21 +241,34 @@ const StageEntry kStageTable[] = {
     FLAGFFT_STAGE2(32, 16, 16, 16, 9, 9),       // 81
     FLAGFFT_STAGE2(16, 16, 8, 16, 15, 9),       // 135
     FLAGFFT_STAGE3(8, 16, 4, 16, 9, 9, 9),      // 729
+    // wider tiles (longer contiguous runs per strided read), picked by the
+    // planner's units-per-workgroup choice
+    FLAGFFT_STAGE2(64, 16, 32, 16, 16, 8),      // 128
+    FLAGFFT_STAGE2(32, 16, 16, 16, 16, 10),     // 160
+    FLAGFFT_STAGE2(32, 16, 16, 16, 16, 16),     // 256
+    FLAGFFT_STAGE3(32, 32, 16, 16, 16, 16, 2),  // 512
+    FLAGFFT_STAGE3(16, 32, 8, 16, 16, 16, 4),   // 1024
+    FLAGFFT_STAGE3(8, 32, 4, 32, 16, 16, 8),    // 2048
 };
 
 }  // namespace
 
-int stage_kernel_id(const std::vector<int> &radices) {
+int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units) {
   const int n = static_cast<int>(sizeof(kStageTable) / sizeof(kStageTable[0]));
+  int first = -1;
   for (int i = 0; i < n; ++i) {
     const StageEntry &e = kStageTable[i];
     if (static_cast<int>(radices.size()) != e.nrad) continue;
     bool same = true;
     for (int j = 0; j < e.nrad; ++j) same = same && radices[static_cast<size_t>(j)] == e.rad[j];
-    if (same) return i;
+    if (!same) continue;
+    if (units <= 0 || (dbl ? e.c64 : e.c32) == units) return i;
+    if (first < 0) first = i;
   }
-  return -1;
+  return units > 0 ? -1 : first;
 }
+
+int stage_kernel_id(const std::vector<int> &radices) { return stage_kernel_id(radices, false, 0); }
 
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept) {
   units_per_group = dbl ? kStageTable[id].c64 : kStageTable[id].c32;

@@ -186,7 +186,13 @@ def test_planner_invariants(n):
         _core_checks(n, _describe(n, t))
 
 
-def test_planner_choices():
+@pytest.fixture
+def untuned(monkeypatch):
+    """The planner's own choices, without rows of the tuned-plan database."""
+    monkeypatch.setenv("FLAGFFT_TUNE_DISABLE", "1")
+
+
+def test_planner_choices(untuned):
     assert "Leaf(n=4096, radices=[16,16,16]" in _describe(4096)
     assert "Rader(p=67" in _describe(67)                      # ctest/test_plan.cpp:47-54
     assert "Rader(p=67" in _describe(67, T.R2C)                # :56-61
@@ -200,7 +206,7 @@ def test_planner_choices():
     assert "TwoDim(n0=4096, n1=4096" in _describe([4096, 4096], T.R2C, rank=2)
 
 
-def test_tuning_overrides_reach_the_planner():
+def test_tuning_overrides_reach_the_planner(untuned):
     try:
         assert ff.set_tuning(radices=[4, 16, 16, 4]) == R.SUCCESS
         assert "radices=[4,16,16,4]" in _describe(4096)
