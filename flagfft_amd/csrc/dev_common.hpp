@@ -44,6 +44,15 @@ __device__ __forceinline__ void stg(GlbC<T> *p, V2<T> v) {
   p->x = v.x;
   p->y = v.y;
 }
+// Element `idx` of a table with a 32-bit index: the address stays a uniform
+// base plus one 32-bit lane offset (global_load with saddr), which keeps many
+// outstanding table loads cheap in registers.
+template <typename T>
+__device__ __forceinline__ V2<T> ldt(const GlbC<T> *base, uint32_t idx) {
+  const FLAGFFT_GLB char *b = reinterpret_cast<const FLAGFFT_GLB char *>(base);
+  return ldg(reinterpret_cast<const GlbC<T> *>(b + idx * static_cast<uint32_t>(sizeof(V2<T>))));
+}
+
 template <typename T>
 __device__ __forceinline__ const GlbC<T> *gc(const void *p) {
   return (const GlbC<T> *)p;

@@ -104,6 +104,7 @@ size_t Step::pack_args(void *src, void *dst, unsigned char *buf) const {
     case Leaf: return put(la);
     case Stage: return put(sa);
     case Mfma: return put(ma);
+    case BluesteinLds: return put(ba);
     case Rader:
     case Bluestein:
     case R2CEven:
@@ -141,6 +142,11 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       s.ma.nseq = scale(ma.nseq);
       s.shape.grid = dim3(static_cast<unsigned>(std::min<int64_t>(blocks(s.ma.nseq, 4), kMfmaGroups)));
       break;
+    case BluesteinLds: {
+      const int64_t nseqs = scale(static_cast<int64_t>(shape.grid.x));
+      s.shape.grid = dim3(static_cast<unsigned>(nseqs));
+      break;
+    }
     case Rader:
     case Bluestein:
     case R2CEven:
@@ -184,6 +190,7 @@ void Executable::resolve_kernels() {
       case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
+      case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
       case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
@@ -893,6 +900,45 @@ void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst,
 
 void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
   const int64_t n = c.n, M = c.m;
+  if (c.lds_conv) {
+    const int id = bluestein_lds_kernel_id(c.radices, dbl());
+    if (id < 0) throw std::runtime_error("no LDS Bluestein instance for this convolution length");
+    Step st;
+    st.kind = Step::BluesteinLds;
+    st.dbl = dbl();
+    st.src = src;
+    st.dst = dst;
+    st.leaf_id = id;
+    BluesteinLdsArgs &a = st.ba;
+    a.gi = s.gi;
+    a.go = s.go;
+    a.ncol = static_cast<int32_t>(s.ncol);
+    a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+    a.n = n;
+    a.tw = roots(M, M, 1);
+    a.chirp = chirp(n);
+    a.bhat = bhat_bluestein(n, M);
+    a.src_mode = m.src_mode & 3;
+    a.dst_mode = m.dst_mode & 3;
+    a.conj_in = m.conj_in;
+    a.conj_out = m.conj_out;
+    a.scale = m.scale;
+    const int ept = bluestein_lds_ept(id);
+    const int tpu = static_cast<int>((M + ept - 1) / ept);
+    const int64_t nseqs = s.nbatch * s.ncol;
+    const int64_t last = (M - 1) + ((M - 1) >> 6);
+    st.ept = ept;
+    st.shape = LaunchShape{dim3(static_cast<unsigned>(nseqs)), dim3(static_cast<unsigned>(tpu)),
+                           static_cast<size_t>(last + 1) * eb()};
+    st.bytes_moved = nseqs * (src_bytes(a.src_mode, n, n, eb()) + dst_bytes(a.dst_mode, n, n, eb()));
+    std::ostringstream lab;
+    lab << "k_bluestein_lds<" << (dbl() ? "fp64" : "fp32") << "," << ept << ">[";
+    for (size_t i = 0; i < c.radices.size(); ++i) lab << (i ? "x" : "") << c.radices[i];
+    lab << "] n=" << n << " M=" << M << " transforms=" << nseqs;
+    st.label = lab.str();
+    ex_.steps.push_back(st);
+    return;
+  }
   if (c.conv && c.conv->passes.size() == 2 && !opt_.generic_kernels && !s.col_fast && s.ncol == 1 &&
       s.gi.es == 1 && s.go.es == 1 && (m.src_mode & 3) == kSrcComplex && (m.dst_mode & 3) == kDstComplex &&
       stage_kernel_id(c.conv->pass_radices[0]) >= 0 && stage_kernel_id(c.conv->pass_radices[1]) >= 0) {

@@ -39,7 +39,7 @@ struct DevBuf {
 enum BufId : int { kBufIn = 0, kBufOut = 1, kBufWs0 = 2 };
 
 struct Step {
-  enum Kind { Pass, Leaf, Stage, Rader, Bluestein, R2CEven, C2REven, RealPost, RealPre, Mfma } kind = Pass;
+  enum Kind { Pass, Leaf, Stage, Rader, Bluestein, BluesteinLds, R2CEven, C2REven, RealPost, RealPre, Mfma } kind = Pass;
   bool dbl = false;
   int ept = 16;
   int leaf_id = -1;  // Leaf / Stage: entry of the specialised kernel table
@@ -51,6 +51,7 @@ struct Step {
   FusedArgs fa{};
   RealPassArgs ra{};
   MfmaArgs ma{};
+  BluesteinLdsArgs ba{};
   int compute = 0;  // Mfma: kComputeBf16x3 / kComputeFp16x3
   LaunchShape shape{};
   int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)

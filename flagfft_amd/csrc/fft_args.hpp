@@ -151,6 +151,22 @@ struct StageArgs {
   const void *chirp, *bhat;
 };
 
+// Bluestein with the whole convolution in LDS (one sequence per workgroup).
+struct BluesteinLdsArgs {
+  const void *in;
+  void *out;
+  Geometry gi, go;
+  int32_t ncol;
+  FastDiv fd_ncol;
+  int64_t n;             // transform length
+  const void *tw;        // w_M^j, j < M
+  const void *chirp;     // n entries
+  const void *bhat;      // FFT_M(b) / M
+  int32_t src_mode, dst_mode;
+  int32_t conj_in, conj_out;
+  double scale;
+};
+
 // Split-precision matrix-core transform of length 1024 (contiguous complex rows).
 enum ComputeMode : int32_t { kComputeVector = 0, kComputeBf16x3 = 1, kComputeFp16x3 = 2 };
 struct MfmaArgs {

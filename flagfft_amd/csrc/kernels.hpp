@@ -49,6 +49,13 @@ const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 
 // QueueArgs argument).
 const void *queue_kernel();
 
+// Bluestein kernels with the convolution in LDS (FP32): id for the
+// convolution's radix sequence (-1 if none), elements per thread, handle.
+int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl);
+int bluestein_lds_ept(int id);
+const void *bluestein_lds_kernel(int id);
+std::vector<std::vector<int>> bluestein_lds_radices();
+
 // Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).
 const void *mfma_kernel(int mode);
 

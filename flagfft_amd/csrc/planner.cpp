@@ -237,6 +237,27 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   double best_cost = 0.0;
   bool have = false;
   const int64_t m0 = 2 * n - 1;
+  if (p == Prec::F32 && m0 > lmax && o.forced_passes.empty()) {
+    // whole convolution in LDS by a specialised instance: one launch
+    int64_t bm = 0;
+    std::vector<int> br;
+    for (const std::vector<int> &r : bluestein_lds_radices()) {
+      int64_t M = 1;
+      for (int x : r) M *= x;
+      if (M >= m0 && M <= 2 * m0 && (bm == 0 || M < bm)) {
+        bm = M;
+        br = r;
+      }
+    }
+    if (bm > 0) {
+      c.kind = CoreKind::Bluestein;
+      c.m = bm;
+      c.radices = br;
+      c.conv.reset();
+      c.lds_conv = true;
+      return c;
+    }
+  }
   for (int64_t M = m0;; ++M) {
     if (have && static_cast<double>(M) >= best_cost) break;
     if (M > 2 * m0 + 64) {
@@ -295,7 +316,8 @@ std::string Core::describe(int indent) const {
         << (compute == 2 ? "fp16" : "bf16") << " hi+lo, products=3, accumulate=fp32)";
       break;
     case CoreKind::Bluestein:
-      o << pad << "Bluestein(n=" << n << ", conv_length=" << m << ", fused=" << (conv ? 0 : 1) << ")\n";
+      o << pad << "Bluestein(n=" << n << ", conv_length=" << m << ", fused=" << (conv ? 0 : 1)
+        << (lds_conv ? ", lds_resident=1" : "") << ")\n";
       if (conv) o << conv->describe(indent + 2);
       else o << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
       break;

@@ -14,7 +14,9 @@ from .helpers import check_accuracy, complex_input, dev_tensor, host, real_input
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [2, 3, 15, 16, 64, 67, 81, 243, 256, 361, 997, 1009, 4096, 6000, 8192, 10007, 16384, 32768, 65536 + 2]
+# even lengths whose half is a Rader (194, 2018) or Bluestein (9998, 65538) prime-factor core
+SIZES = [2, 3, 15, 16, 64, 67, 81, 194, 243, 256, 361, 997, 1009, 2018, 4096, 6000, 8192, 9998, 10007, 16384, 32768,
+         65536 + 2]
 
 
 def _r2c(dev, x, n, batch, dbl):
