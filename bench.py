@@ -193,6 +193,59 @@ def measure(cases, steps, warmup, torch, ff, stream):
     return (t1 - t0), launches, per_case
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_per_kernel.json")
+
+
+def traffic_of(symbol: str):
+    """HBM bytes per dispatch of `symbol` from the committed rocprofv3 counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    row = table.get("kernels", {}).get(symbol)
+    return None if row is None else int(row["hbm_bytes_per_dispatch"])
+
+
+def counter_means(csv_path: str, counter: str):
+    """Mean value of `counter` per kernel symbol in a rocprofv3 counter_collection.csv."""
+    import csv
+
+    sums, counts = {}, {}
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("flagfft::dev::", "")
+            sums[name] = sums.get(name, 0.0) + float(r["Counter_Value"])
+            counts[name] = counts.get(name, 0) + 1
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def write_traffic(args):
+    """Combine FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch) of the bench and of a
+    plain copy of known size into per-kernel HBM bytes per dispatch."""
+    fetch = counter_means(args.fetch_csv, "FETCH_SIZE")
+    write = counter_means(args.write_csv, "WRITE_SIZE")
+    cf = counter_means(args.copy_fetch_csv, "FETCH_SIZE")
+    cw = counter_means(args.copy_write_csv, "WRITE_SIZE")
+    copy_name = max(cf, key=lambda k: cf[k])  # the calibration copy dominates its run
+    half = args.copy_bytes / 2  # the copy reads and writes copy_bytes / 2 each
+    f_factor = half / (cf[copy_name] * 1024.0)
+    w_factor = half / (cw[copy_name] * 1024.0)
+    kernels = {}
+    for k in fetch:
+        if k in write and k.startswith(("k_", "flagfft")):
+            kernels[k] = {"fetch_kib": round(fetch[k], 1), "write_kib": round(write[k], 1),
+                          "hbm_bytes_per_dispatch": int(fetch[k] * 1024 * f_factor + write[k] * 1024 * w_factor)}
+    out = {"fetch_factor": round(f_factor, 4), "write_factor": round(w_factor, 4), "calibration_kernel": copy_name,
+           "calibration_bytes": args.copy_bytes, "kernels": kernels}
+    with open(args.write_traffic, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps({"fetch_factor": out["fetch_factor"], "write_factor": out["write_factor"],
+                      "kernels": len(kernels)}))
+
+
 def roofline_of(launches):
     """Dominant kernel symbol by total device time; achieved = bytes / mean duration."""
     by_sym = {}
@@ -209,7 +262,7 @@ def roofline_of(launches):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(sym),
         "kernel": sym, "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": int(bytes_per_launch),
         "launches": d["calls"], "share_of_device_time": round(d["ms"] / sum(v["ms"] for v in by_sym.values()), 3),
     }, {k: {"ms": round(v["ms"], 3), "calls": v["calls"], "avg_us": round(v["ms"] / max(v["calls"], 1) * 1e3, 2)}
@@ -368,13 +421,32 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--suite", action="store_true", help="also measure the other configurations (stderr)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes 20x "
+                    "(counter calibration run under rocprofv3)")
+    ap.add_argument("--write-traffic", help="write the per-kernel HBM traffic summary to this file and exit")
+    ap.add_argument("--fetch-csv")
+    ap.add_argument("--write-csv")
+    ap.add_argument("--copy-fetch-csv")
+    ap.add_argument("--copy-write-csv")
+    ap.add_argument("--copy-bytes", type=int, default=0)
     args = ap.parse_args()
+    if args.write_traffic:
+        write_traffic(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the path shards by batch with no communication; see the design notes)", file=sys.stderr)
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe // 2, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        return
 
     from flagfft_amd import flagfft as ff
 

@@ -190,7 +190,7 @@ void Executable::resolve_kernels() {
       case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
-      case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id); break;
+      case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
       case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
@@ -909,6 +909,7 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     st.src = src;
     st.dst = dst;
     st.leaf_id = id;
+    st.leaf_mode = opt_.forced_units == 1 ? 1 : 0;  // tuning: prefetched base twiddles
     BluesteinLdsArgs &a = st.ba;
     a.gi = s.gi;
     a.go = s.go;
@@ -932,7 +933,8 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
                            static_cast<size_t>(last + 1) * eb()};
     st.bytes_moved = nseqs * (src_bytes(a.src_mode, n, n, eb()) + dst_bytes(a.dst_mode, n, n, eb()));
     std::ostringstream lab;
-    lab << "k_bluestein_lds<" << (dbl() ? "fp64" : "fp32") << "," << ept << ">[";
+    lab << "k_bluestein_lds<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << (st.leaf_mode ? "true" : "false")
+        << ">[";
     for (size_t i = 0; i < c.radices.size(); ++i) lab << (i ? "x" : "") << c.radices[i];
     lab << "] n=" << n << " M=" << M << " transforms=" << nseqs;
     st.label = lab.str();

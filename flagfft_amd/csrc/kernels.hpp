@@ -53,7 +53,7 @@ const void *queue_kernel();
 // convolution's radix sequence (-1 if none), elements per thread, handle.
 int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl);
 int bluestein_lds_ept(int id);
-const void *bluestein_lds_kernel(int id);
+const void *bluestein_lds_kernel(int id, int variant);  // variant 1: prefetched base twiddles
 std::vector<std::vector<int>> bluestein_lds_radices();
 
 // Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).

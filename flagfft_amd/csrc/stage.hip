@@ -37,10 +37,11 @@ __device__ __forceinline__ V2<T> tw_power(const V2<T> *p) {
   return r;
 }
 
-// kTwPow: inner-stage twiddles w^(j k) from one table load w^k per butterfly
-// (binary powering) instead of R - 1 loads; for register-bound kernels.
-template <typename T, int E, int TPU, typename RS, bool kLdsIn, int PS = 5, bool kTwPow = false, typename Load,
-          typename Sink>
+// kTwPow: inner-stage twiddles w^(j k) by binary powers of the butterfly's
+// base twiddle w^k, which twr holds (prefetch_base_twiddles), instead of R - 1
+// table loads; for register-bound kernels.
+template <typename T, int E, int TPU, typename RS, bool kLdsIn, int PS = 5, bool kTwPow = false,
+          bool kPreBase = false, typename Load, typename Sink>
 __device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr, const GlbC<T> *tws, bool sink_lds,
                                            Load load0, Sink sink) {
   auto lpad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
@@ -71,10 +72,13 @@ __device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr
 #pragma unroll
           for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], twr[off + m * (RR - 1) + (j - 1)]);
         } else if constexpr (kTwPow) {
-          constexpr int ts = R / (q * RR);
-          const uint32_t kk = static_cast<uint32_t>(b % q);
           V2<T> pw[4];
-          pw[0] = ldt(tws, kk * ts);
+          if constexpr (kPreBase) {
+            pw[0] = twr[RS::template pow_offset<E>(s) + m];  // prefetched base twiddle w^k
+          } else {
+            constexpr int ts = R / (q * RR);
+            pw[0] = ldt(tws, static_cast<uint32_t>(b % q) * ts);
+          }
           pw[1] = cmul(pw[0], pw[0]);
           pw[2] = cmul(pw[1], pw[1]);
           pw[3] = cmul(pw[2], pw[2]);
@@ -124,7 +128,7 @@ __device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr
 //   y[k] = conj(W[k]) * chirp[k], k < n           (src/exec/raw_nodes.cpp:153-204)
 // One pad slot every 64 elements so M = 20160 fits the 160 KiB of LDS; up to
 // 1024 lanes per workgroup keep the per-lane element count (and registers) low.
-template <typename T, int E, int... Rs>
+template <typename T, int E, bool kPreBase, int... Rs>
 __global__ __launch_bounds__(1024) void k_bluestein_lds(const BluesteinLdsArgs a) {
   using RS = RadixSeq<Rs...>;
   constexpr int M = RS::kN;
@@ -140,17 +144,20 @@ __global__ __launch_bounds__(1024) void k_bluestein_lds(const BluesteinLdsArgs a
   const GlbC<T> *tws = gc<T>(a.tw);
   const GlbC<T> *bhat = gc<T>(a.bhat);
   const T scale = static_cast<T>(a.scale);
-  V2<T> twr[1];  // unused: twiddles come from binary powers of one load per butterfly
+  // kPreBase: one base twiddle per inner-stage butterfly loaded up front and
+  // shared by both transforms (more registers), else one load per stage
+  V2<T> twr[kPreBase ? RS::template pow_total<E>() : 1];
+  if constexpr (kPreBase) prefetch_base_twiddles<T, E, TPU, RS>(twr, tws, t);
   auto pad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
   auto to_lds = [&](int idx, V2<T> v) __attribute__((always_inline)) { stc(buf + pad(idx), v); };
-  run_stages<T, E, TPU, RS, false, PS, true>(
+  run_stages<T, E, TPU, RS, false, PS, true, kPreBase>(
       buf, t, twr, tws, true,
       [&](int e) __attribute__((always_inline)) {
         return src_elem<T>(a.in, ibase, a.gi.es, a.src_mode | kSrcChirpFlag, a.n, a.chirp, a.conj_in, e);
       },
       to_lds);
   __syncthreads();
-  run_stages<T, E, TPU, RS, true, PS, true>(
+  run_stages<T, E, TPU, RS, true, PS, true, kPreBase>(
       buf, t, twr, tws, false,
       [&](int e) __attribute__((always_inline)) { return cconj(cmul(ldc(buf + pad(e)), ldg(bhat + e))); },
       [&](int idx, V2<T> v) __attribute__((always_inline)) {
@@ -329,13 +336,16 @@ struct BluesteinLdsEntry {
   int nrad;
   int rad[4];
   int e32;
-  const void *f32;
+  const void *f32;      // base twiddles loaded per stage
+  const void *f32_pre;  // base twiddles prefetched at kernel start
 };
 
 // FP32 only: the FP64 convolution of these lengths does not fit in LDS.
 const BluesteinLdsEntry kBluesteinLdsTable[] = {
-    {4, {16, 16, 16, 4}, 16, (const void *)dev::k_bluestein_lds<float, 16, 16, 16, 16, 4>},   // 16384
-    {4, {14, 12, 12, 10}, 20, (const void *)dev::k_bluestein_lds<float, 20, 14, 12, 12, 10>},  // 20160
+    {4, {16, 16, 16, 4}, 16, (const void *)dev::k_bluestein_lds<float, 16, false, 16, 16, 16, 4>,
+     (const void *)dev::k_bluestein_lds<float, 16, true, 16, 16, 16, 4>},  // 16384
+    {4, {14, 12, 12, 10}, 20, (const void *)dev::k_bluestein_lds<float, 20, false, 14, 12, 12, 10>,
+     (const void *)dev::k_bluestein_lds<float, 20, true, 14, 12, 12, 10>},  // 20160
 };
 
 }  // namespace
@@ -354,7 +364,9 @@ int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl) {
 }
 
 int bluestein_lds_ept(int id) { return kBluesteinLdsTable[id].e32; }
-const void *bluestein_lds_kernel(int id) { return kBluesteinLdsTable[id].f32; }
+const void *bluestein_lds_kernel(int id, int variant) {
+  return variant == 1 ? kBluesteinLdsTable[id].f32_pre : kBluesteinLdsTable[id].f32;
+}
 // Lengths with an instance, ascending (the planner's candidates).
 std::vector<std::vector<int>> bluestein_lds_radices() {
   std::vector<std::vector<int>> out;
