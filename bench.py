@@ -4,12 +4,14 @@ Default workload (arbitrary-length 1D, one "step" = one pass of the hot path
 over one batch of synthetic input):
   C2C f32 and Z2Z f64 forward at N=10007 (prime) and N=10935 = 3^7*5, batch 64, N(0,1);
   R2C->C2R f32 and D2Z->Z2D f64 at N=6000, batch 128, U[-1,1].
-Every transform goes through the C ABI (libflagfft.so) on one HIP stream.
+Every transform goes through the C ABI (libflagfft.so).
 Inputs rotate over buffer sets larger than 2x the 256 MB Infinity Cache so
 the timed region reads HBM.
 
 value          algorithmic HBM GB/s of the whole step: sum over transforms of
-               batch * (input bytes + output bytes), divided by wall time
+               batch * (input bytes + output bytes), divided by wall time, each
+               transform family on its own HIP stream (they are independent);
+               config.single_stream_gbps is the same step on one stream
 roofline       dominant kernel (by total device time): algorithmic bytes per
                launch / mean launch duration from HIP events recorded on the
                plan's stream around every launch of a second timed region of
@@ -44,8 +46,9 @@ def log(*a):
 class Case:
     """One transform family of the workload: plans, rotating device buffers, one exec."""
 
-    def __init__(self, name, kind, n, batch, dbl, dev, ff, torch):
+    def __init__(self, name, kind, n, batch, dbl, dev, ff, torch, compute=0):
         self.name, self.kind, self.n, self.batch, self.dbl = name, kind, n, batch, dbl
+        self.compute = compute
         self.ff, self.torch, self.dev = ff, torch, dev
         cdt = torch.complex128 if dbl else torch.complex64
         rdt = torch.float64 if dbl else torch.float32
@@ -53,7 +56,7 @@ class Case:
         h = n // 2 + 1
         T = ff.Type
         if kind == "c2c":
-            self.plans = [ff.Plan.one_d(n, T.Z2Z if dbl else T.C2C, batch)]
+            self.plans = [ff.Plan.one_d_compute(n, T.Z2Z if dbl else T.C2C, batch, compute)]
             self.alg_bytes = batch * n * esz * 2
             self.shapes = [((batch, n), cdt), ((batch, n), cdt)]
         else:  # real roundtrip R2C -> C2R
@@ -130,6 +133,8 @@ SUITE = [  # the other configurations, reported with --suite (not the headline l
     [("C2C f32 N=2^20 x16", "c2c", 1 << 20, 16, False)],
     [("C2C f32 N=2^24 x1", "c2c", 1 << 24, 1, False)],
     [("C2C f32 N=1024 x65536", "c2c", 1024, 65536, False)],
+    [("C2C f32 N=1024 x65536 FP16 hi+lo matrix cores", "c2c", 1024, 65536, False, 2)],
+    [("C2C f32 N=1024 x65536 BF16 hi+lo matrix cores", "c2c", 1024, 65536, False, 1)],
     [("R2C->C2R f32 N=2^20 x16", "real", 1 << 20, 16, False)],
 ]
 
@@ -147,37 +152,47 @@ def kernel_symbol(label: str) -> str:
         return fam
     targs, _, rad = rest.partition(">")
     parts = targs.split(",")
-    parts[0] = "float" if parts[0] == "fp32" else "double"
+    parts[0] = {"fp32": "float", "fp64": "double"}.get(parts[0], parts[0])
     if rad.startswith("["):
         parts += rad.strip("[]").split("x")
     return f"{fam}<{', '.join(parts)}>"
 
 
-def measure(cases, steps, warmup, torch, ff, stream):
-    """Two timed regions of the same K steps on one stream:
-    1. throughput: nothing but the launches between the synchronisations (-> value);
-    2. per-launch HIP events recorded on the stream around every launch (-> roofline)."""
-    for c in cases:
-        c.bind(stream.cuda_stream)
-    with torch.cuda.stream(stream):
-        for s in range(warmup):
-            for c in cases:
-                c.run(s)
+def measure(cases, steps, warmup, torch, ff, stream, case_streams):
+    """Three timed regions of the same K steps:
+    1. every transform family on its own HIP stream, launches only (-> value);
+    2. all families on one stream, launches only (-> single_stream_gbps);
+    3. one stream with HIP events recorded around every launch (-> roofline)."""
+
+    def bind(per_case):
+        for c, st in zip(cases, case_streams if per_case else [stream] * len(cases)):
+            c.stream = st
+            c.bind(st.cuda_stream)
+
+    def region(first_step):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(steps):
             for c in cases:
-                c.run(warmup + s)
+                with torch.cuda.stream(c.stream):
+                    c.run(first_step + s)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for c in cases:  # per-launch events from here on
-            for p in c.plans:
-                ff.check("flagfftxSetProfiling", ff.set_profiling(p.handle, True))
-        torch.cuda.synchronize()
-        for s in range(steps):
-            for c in cases:
-                c.run(warmup + steps + s)
-        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    bind(True)
+    for s in range(warmup):
+        for c in cases:
+            with torch.cuda.stream(c.stream):
+                c.run(s)
+    t_multi = region(warmup)
+    bind(False)
+    for c in cases:
+        c.run(0)
+    t_single = region(warmup + steps)
+    for c in cases:  # per-launch events from here on
+        for p in c.plans:
+            ff.check("flagfftxSetProfiling", ff.set_profiling(p.handle, True))
+    region(warmup + 2 * steps)
     launches = {}
     per_case = {}
     for c in cases:
@@ -190,7 +205,7 @@ def measure(cases, steps, warmup, torch, ff, stream):
                 case_ms += ms
         per_case[c.name] = {"alg_bytes": c.alg_bytes, "device_ms_per_exec": case_ms / steps,
                             "gbps": c.alg_bytes / (case_ms / steps * 1e-3) / 1e9 if case_ms > 0 else None}
-    return (t1 - t0), launches, per_case
+    return t_multi, t_single, launches, per_case
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_per_kernel.json")
@@ -395,20 +410,35 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
     return out
 
 
+def accuracy_sample(case, torch, rows: int = 64):
+    """Relative L2 error of the first rows of one exec against the FP64 oracle."""
+    from oracle import oracle as ora
+
+    b = case.sets[0]
+    case.run(0)
+    torch.cuda.synchronize()
+    x = b[0][:rows].cpu().numpy().astype(np.complex128)
+    y = b[1][:rows].cpu().numpy()
+    return float(f"{ora.rel_l2(y, ora.fft(x))[0]:.3e}")
+
+
 def run_workload(spec, args, torch, ff, dev, stream, seed):
-    cases = [Case(name, kind, n, batch, dbl, dev, ff, torch) for name, kind, n, batch, dbl in spec]
+    # one stream per transform family for the throughput region (default priority)
+    cases = [Case(*row[:5], dev, ff, torch, *row[5:]) for row in spec]
     footprint = sum(c.footprint() for c in cases)
     nsets = max(1, -(-2 * MALL_BYTES // max(footprint, 1)) + 1)
     nsets = min(nsets, 64)
     for i, c in enumerate(cases):
         c.allocate(nsets, seed + i)
     torch.cuda.synchronize()
-    t, launches, per_case = measure(cases, args.steps, args.warmup, torch, ff, stream)
+    case_streams = [torch.cuda.Stream(device=dev) for _ in cases]
+    t, t1, launches, per_case = measure(cases, args.steps, args.warmup, torch, ff, stream, case_streams)
     alg = sum(c.alg_bytes for c in cases)
     value = args.steps * alg / t / 1e9
     roof, kernels = roofline_of(launches)
     pass_bytes = sum(nb * calls for (ms, calls, nb) in launches.values())
     return cases, {"value": value, "ms_per_step": t / args.steps * 1e3, "alg_bytes_per_step": alg,
+                   "single_stream_gbps": args.steps * alg / t1 / 1e9, "streams": len(case_streams),
                    "pass_gbps": pass_bytes / t / 1e9, "roofline": roof, "kernels": kernels,
                    "per_case": per_case, "buffer_sets": nsets, "footprint_mb": round(nsets * footprint / 2**20)}
 
@@ -483,7 +513,8 @@ def main():
             "hbm_frac_of_peak": round(res["value"] / HBM_PEAK_GBS, 4),
             "pass_bytes_gbps": round(res["pass_gbps"], 2),
             "input_rotation": f"{res['buffer_sets']} buffer sets, {res['footprint_mb']} MiB",
-            "parallelism": "single GPU, one HIP stream",
+            "parallelism": f"single GPU, one HIP stream per transform family ({res['streams']} streams)",
+            "single_stream_gbps": round(res["single_stream_gbps"], 2),
         },
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
@@ -497,7 +528,8 @@ def main():
             del cases
             torch.cuda.empty_cache()
             cases, r = run_workload(spec, args, torch, ff, dev, stream, 99)
-            log(json.dumps({"suite": spec[0][0], "gbps": round(r["value"], 1),
+            err = accuracy_sample(cases[0], torch) if cases[0].kind == "c2c" else None
+            log(json.dumps({"suite": spec[0][0], "gbps": round(r["value"], 1), "rel_l2_vs_fp64": err,
                             "frac": round(r["value"] / HBM_PEAK_GBS, 3), "us_per_exec": round(r["ms_per_step"] * 1e3, 2),
                             "roofline": r["roofline"]}))
     print(json.dumps(line), flush=True)

@@ -892,7 +892,7 @@ void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst,
   st.shape = LaunchShape{dim3(static_cast<unsigned>(groups)), dim3(256), 0};
   st.bytes_moved = s.nbatch * 1024 * 8 * 2;
   std::ostringstream lab;
-  lab << "k_mfma1024<" << (c.compute == kComputeFp16x3 ? "fp16x3" : "bf16x3") << "> R=1024 transforms=" << s.nbatch
+  lab << "k_mfma1024<" << (c.compute == kComputeFp16x3 ? "true> fp16x3" : "false> bf16x3") << " R=1024 transforms=" << s.nbatch
       << " groups=" << groups;
   st.label = lab.str();
   ex_.steps.push_back(st);
