@@ -91,7 +91,7 @@ hipError_t Executable::harvest_profile() const {
   return first_err;
 }
 
-size_t Step::pack_args(void *src, void *dst, unsigned char *buf) const {
+size_t Step::pack_args(void *src, void *dst, unsigned char *buf, void *aux) const {
   auto put = [&](auto a) {
     static_assert(sizeof(a) <= kArgBytes, "kernel argument struct too large");
     a.in = src;
@@ -104,7 +104,12 @@ size_t Step::pack_args(void *src, void *dst, unsigned char *buf) const {
     case Leaf: return put(la);
     case Stage: return put(sa);
     case Mfma: return put(ma);
-    case BluesteinLds: return put(ba);
+    case BluesteinLds: {
+      BluesteinLdsArgs b = ba;
+      b.park = aux;
+      return put(b);
+    }
+    case BluesteinCombine: return put(ca);
     case Rader:
     case Bluestein:
     case R2CEven:
@@ -142,11 +147,10 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       s.ma.nseq = scale(ma.nseq);
       s.shape.grid = dim3(static_cast<unsigned>(std::min<int64_t>(blocks(s.ma.nseq, 4), kMfmaGroups)));
       break;
-    case BluesteinLds: {
-      const int64_t nseqs = scale(static_cast<int64_t>(shape.grid.x));
-      s.shape.grid = dim3(static_cast<unsigned>(nseqs));
+    case BluesteinLds:
+    case BluesteinCombine:  // workgroups per sequence are fixed
+      s.shape.grid = dim3(static_cast<unsigned>(scale(static_cast<int64_t>(shape.grid.x))));
       break;
-    }
     case Rader:
     case Bluestein:
     case R2CEven:
@@ -175,7 +179,7 @@ hipError_t Executable::run_batch(const void *in, void *out, void *const *wsp, in
   };
   for (const Step &s0 : steps) {
     const Step s = s0.rebatched(batch, built_batch);
-    s.pack_args(res(s.src), res(s.dst), buf);
+    s.pack_args(res(s.src), res(s.dst), buf, s.aux >= 0 ? res(s.aux) : nullptr);
     void *kp[] = {buf};
     hipError_t e = hipLaunchKernel(s.func, s.shape.grid, s.shape.block, kp, s.shape.lds_bytes, st);
     if (e != hipSuccess) return e;
@@ -191,6 +195,7 @@ void Executable::resolve_kernels() {
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
+      case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
       case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
@@ -219,7 +224,7 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
   alignas(16) unsigned char buf[Step::kArgBytes];
   for (size_t si = 0; si < steps.size(); ++si) {
     const Step &s = steps[si];
-    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf);
+    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf, s.aux >= 0 ? buffer(s.aux, in, out) : nullptr);
     void *kp[] = {buf};
     hipError_t e = hipSuccess;
     EvPair ev;
@@ -242,7 +247,7 @@ hipError_t Executable::run(const void *in, void *out, hipStream_t st) const {
 hipError_t Executable::run_graph(const void *in, void *out, hipStream_t st) {
   alignas(16) unsigned char buf[Step::kArgBytes];
   auto params = [&](const Step &s, void **kp) {
-    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf);
+    s.pack_args(buffer(s.src, in, out), buffer(s.dst, in, out), buf, s.aux >= 0 ? buffer(s.aux, in, out) : nullptr);
     kp[0] = buf;
     hipKernelNodeParams p{};
     p.func = const_cast<void *>(s.func);
@@ -354,6 +359,14 @@ void Builder::finish() {
   temp_execs_.clear();
 }
 
+int Builder::cu_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    n = 256;
+  return n;
+}
+
 int Builder::pick_ws(std::initializer_list<int> avoid, size_t bytes) {
   for (int id = kBufWs0;; ++id) {
     if (std::find(avoid.begin(), avoid.end(), id) != avoid.end()) continue;
@@ -421,6 +434,36 @@ const void *Builder::bhat_bluestein(int64_t n, int64_t M) {
   auto b64 = std::make_shared<DevBuf>(static_cast<size_t>(M) * 16);
   hip_check(gen_bluestein_b(static_cast<double2 *>(b64->ptr), n, M, st_), "gen_bluestein_b");
   return fp64_transform_then_cast(k, b64, M, 1.0 / static_cast<double>(M));
+}
+
+// FFT_M(b) / M with entry k moved to slot P(k) = sum_s digit_s(k) * m_s
+// (digits of k in the mixed radix of the stages, least significant first;
+// m_s = M / (r_0 ... r_s)): the order a decimation-in-frequency transform
+// leaves its spectrum in.
+const void *Builder::bhat_bluestein_digit_reversed(int64_t n, int64_t M, const std::vector<int> &radices) {
+  std::string k = "bhatBrev:" + std::to_string(eb()) + ":" + std::to_string(n) + ":" + std::to_string(M);
+  for (int r : radices) k += ":" + std::to_string(r);
+  auto it = ex_.tables.find(k);
+  if (it != ex_.tables.end()) return it->second->ptr;
+  const void *nat = bhat_bluestein(n, M);
+  const size_t eb_ = static_cast<size_t>(eb());
+  std::vector<unsigned char> h(static_cast<size_t>(M) * eb_), p(static_cast<size_t>(M) * eb_);
+  hip_check(hipMemcpyAsync(h.data(), nat, h.size(), hipMemcpyDeviceToHost, st_), "bhat download");
+  hip_check(hipStreamSynchronize(st_), "bhat download");
+  for (int64_t kk = 0; kk < M; ++kk) {
+    int64_t pos = 0, w = 1, span = M;
+    for (int r : radices) {
+      span /= r;
+      pos += ((kk / w) % r) * span;
+      w *= r;
+    }
+    std::memcpy(&p[static_cast<size_t>(pos) * eb_], &h[static_cast<size_t>(kk) * eb_], eb_);
+  }
+  auto buf = std::make_shared<DevBuf>(p.size());
+  hip_check(hipMemcpyAsync(buf->ptr, p.data(), p.size(), hipMemcpyHostToDevice, st_), "bhat upload");
+  hip_check(hipStreamSynchronize(st_), "bhat upload");
+  ex_.tables[k] = buf;
+  return buf->ptr;
 }
 
 const void *Builder::rader_idx(int64_t p, int64_t g) {
@@ -898,10 +941,115 @@ void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst,
   ex_.steps.push_back(st);
 }
 
+// Convolution of length M over `split` workgroups per sequence: the split
+// kernel convolves block h of the radix-split first stage and parks it, the
+// combine kernel applies the radix-split last stage, the chirp and the sink.
+void Builder::emit_bluestein_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m, int split) {
+  const int64_t n = c.n, M = c.m;
+  const int id = bluestein_split_kernel_id(M, dbl(), split);
+  const std::vector<int> rad = bluestein_lds_kernel_radices(id);
+  const int64_t ml = M / split;
+  if (2 * n > M) throw std::runtime_error("split LDS Bluestein needs n <= M / 2");
+  const int64_t nseqs = s.nbatch * s.ncol;
+  const int park = pick_ws({src, dst}, static_cast<size_t>(nseqs * M) * eb());
+  std::vector<int> digits = rad;
+  digits.insert(digits.begin(), split);
+  const void *tw = roots(M, M, 1);
+  const void *ch = chirp(n);
+  Step st;
+  st.kind = Step::BluesteinLds;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  st.aux = park;
+  st.split = split;
+  st.leaf_id = id;
+  st.leaf_mode = 0;
+  BluesteinLdsArgs &a = st.ba;
+  a.gi = s.gi;
+  a.go = s.go;
+  a.ncol = static_cast<int32_t>(s.ncol);
+  a.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  a.n = n;
+  a.tw = tw;
+  a.chirp = ch;
+  a.bhat = bhat_bluestein_digit_reversed(n, M, digits);
+  a.src_mode = m.src_mode & 3;
+  a.dst_mode = m.dst_mode & 3;
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  const int tpu = bluestein_lds_threads(id, 0);
+  const int64_t last = (ml - 1) + ((ml - 1) >> 6);
+  st.ept = static_cast<int>((ml + tpu - 1) / tpu);
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nseqs * split)), dim3(static_cast<unsigned>(tpu)),
+                         static_cast<size_t>(last + 1) * eb()};
+  st.bytes_moved = nseqs * src_bytes(a.src_mode, n, n, eb());
+  const char *pr = dbl() ? "fp64" : "fp32";
+  std::ostringstream lab;
+  lab << "k_bluestein_split<" << pr << "," << tpu << "," << split << ">[";
+  for (size_t i = 0; i < rad.size(); ++i) lab << (i ? "x" : "") << rad[i];
+  lab << "] n=" << n << " M=" << M << " transforms=" << nseqs;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+
+  Step cb;
+  cb.kind = Step::BluesteinCombine;
+  cb.dbl = dbl();
+  cb.src = park;
+  cb.dst = dst;
+  cb.split = split;
+  BluesteinCombineArgs &b = cb.ca;
+  b.go = s.go;
+  b.ncol = static_cast<int32_t>(s.ncol);
+  b.fd_ncol = make_fastdiv(static_cast<uint32_t>(s.ncol));
+  b.nb = static_cast<int32_t>((n + 255) / 256);
+  b.fd_nb = make_fastdiv(static_cast<uint32_t>(b.nb));
+  b.n = static_cast<int32_t>(n);
+  b.ml = static_cast<int32_t>(ml);
+  b.tw = tw;
+  b.chirp = ch;
+  b.dst_mode = m.dst_mode & 3;
+  b.conj_out = m.conj_out;
+  b.scale = m.scale;
+  cb.shape = LaunchShape{dim3(static_cast<unsigned>(nseqs * b.nb)), dim3(256), 0};
+  cb.bytes_moved = nseqs * dst_bytes(b.dst_mode, n, n, eb());
+  std::ostringstream lb;
+  lb << "k_bluestein_combine<" << pr << "," << split << "> n=" << n << " M=" << M << " transforms=" << nseqs;
+  cb.label = lb.str();
+  ex_.steps.push_back(cb);
+}
+
 void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
   const int64_t n = c.n, M = c.m;
   if (c.lds_conv) {
-    const int id = bluestein_lds_kernel_id(c.radices, dbl());
+    const int64_t nseqs_all = s.nbatch * s.ncol;
+    // Few sequences: spread each convolution over 2 or 4 workgroups (blocks
+    // of a radix-P first stage) plus a combine pass, so the launch fills the
+    // GPU.  Tuning (units): 5 one workgroup per sequence, 6 split in 2, 7
+    // split in 4.
+    int split = 0;
+    if (opt_.forced_units == 6) {
+      split = 2;
+    } else if (opt_.forced_units == 7) {
+      split = 4;
+    } else if (opt_.forced_units < 1 || opt_.forced_units > 5) {
+      // measured on MI355X: a whole convolution per workgroup stays ahead
+      // until fewer than 1/8 of the CUs have a sequence; the two-halves form
+      // (two convolutions in a row per workgroup) already below 1/2
+      const int ncu = cu_count();
+      if (c.lds_halves ? nseqs_all * 4 <= ncu : nseqs_all * 8 <= ncu)
+        split = 4;
+      else if (c.lds_halves && nseqs_all * 2 <= ncu)
+        split = 2;
+    }
+    if (split > 0 && bluestein_split_kernel_id(M, dbl(), split) < 0) split = split == 4 ? 2 : 0;
+    if (split > 0 && bluestein_split_kernel_id(M, dbl(), split) < 0) split = 0;
+    if (split > 0) {
+      emit_bluestein_split(c, s, src, dst, m, split);
+      return;
+    }
+    const int id = bluestein_lds_kernel_id(c.radices, dbl(), c.lds_halves);
     if (id < 0) throw std::runtime_error("no LDS Bluestein instance for this convolution length");
     Step st;
     st.kind = Step::BluesteinLds;
@@ -909,7 +1057,16 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     st.src = src;
     st.dst = dst;
     st.leaf_id = id;
-    st.leaf_mode = opt_.forced_units == 1 ? 1 : 0;  // tuning: prefetched base twiddles
+    // tuning (units), FP32 instances with Stockham kernels: 1 Stockham with
+    // prefetched base twiddles, 2 Stockham, 3 in-place with table twiddles,
+    // 4 (default) in-place with binary-powered twiddles
+    int variant = 3;
+    if (bluestein_lds_has_stockham(id))
+      variant = opt_.forced_units == 1 ? 1 : opt_.forced_units == 2 ? 0 : opt_.forced_units == 3 ? 2 : 3;
+    const bool in_place = variant >= 2;
+    if (in_place && 2 * n > M) throw std::runtime_error("in-place LDS Bluestein needs n <= M / 2");
+    st.leaf_mode = variant;
+    const int64_t nseqs = s.nbatch * s.ncol;
     BluesteinLdsArgs &a = st.ba;
     a.gi = s.gi;
     a.go = s.go;
@@ -918,23 +1075,34 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     a.n = n;
     a.tw = roots(M, M, 1);
     a.chirp = chirp(n);
-    a.bhat = bhat_bluestein(n, M);
+    if (!in_place) {
+      a.bhat = bhat_bluestein(n, M);
+    } else {
+      std::vector<int> digits = c.radices;
+      if (c.lds_halves) digits.insert(digits.begin(), 2);
+      a.bhat = bhat_bluestein_digit_reversed(n, M, digits);
+    }
+    if (c.lds_halves) st.aux = pick_ws({src, dst}, static_cast<size_t>(nseqs * n) * eb());
     a.src_mode = m.src_mode & 3;
     a.dst_mode = m.dst_mode & 3;
     a.conj_in = m.conj_in;
     a.conj_out = m.conj_out;
     a.scale = m.scale;
-    const int ept = bluestein_lds_ept(id);
-    const int tpu = static_cast<int>((M + ept - 1) / ept);
-    const int64_t nseqs = s.nbatch * s.ncol;
-    const int64_t last = (M - 1) + ((M - 1) >> 6);
-    st.ept = ept;
+    const int tpu = bluestein_lds_threads(id, variant);
+    const int64_t ml = c.lds_halves ? M / 2 : M;  // points held in LDS
+    const int64_t last = (ml - 1) + ((ml - 1) >> 6);
+    st.ept = in_place ? static_cast<int>((ml + tpu - 1) / tpu) : bluestein_lds_ept(id);
     st.shape = LaunchShape{dim3(static_cast<unsigned>(nseqs)), dim3(static_cast<unsigned>(tpu)),
                            static_cast<size_t>(last + 1) * eb()};
     st.bytes_moved = nseqs * (src_bytes(a.src_mode, n, n, eb()) + dst_bytes(a.dst_mode, n, n, eb()));
     std::ostringstream lab;
-    lab << "k_bluestein_lds<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << (st.leaf_mode ? "true" : "false")
-        << ">[";
+    const char *pr = dbl() ? "fp64" : "fp32";
+    if (c.lds_halves)
+      lab << "k_bluestein_halves<" << pr << "," << tpu << ">[";
+    else if (in_place)
+      lab << "k_bluestein_ip<" << pr << "," << tpu << "," << (variant == 2 ? "true" : "false") << ">[";
+    else
+      lab << "k_bluestein_lds<" << pr << "," << st.ept << "," << (variant == 1 ? "true" : "false") << ">[";
     for (size_t i = 0; i < c.radices.size(); ++i) lab << (i ? "x" : "") << c.radices[i];
     lab << "] n=" << n << " M=" << M << " transforms=" << nseqs;
     st.label = lab.str();

@@ -39,12 +39,27 @@ struct DevBuf {
 enum BufId : int { kBufIn = 0, kBufOut = 1, kBufWs0 = 2 };
 
 struct Step {
-  enum Kind { Pass, Leaf, Stage, Rader, Bluestein, BluesteinLds, R2CEven, C2REven, RealPost, RealPre, Mfma } kind = Pass;
+  enum Kind {
+    Pass,
+    Leaf,
+    Stage,
+    Rader,
+    Bluestein,
+    BluesteinLds,
+    BluesteinCombine,
+    R2CEven,
+    C2REven,
+    RealPost,
+    RealPre,
+    Mfma
+  } kind = Pass;
   bool dbl = false;
   int ept = 16;
   int leaf_id = -1;  // Leaf / Stage: entry of the specialised kernel table
   int leaf_mode = 0;  // Leaf: 0 complex rows, 1 R2C, 2 C2R
   int src = kBufIn, dst = kBufOut;
+  int aux = -1;  // BluesteinLds halves / split: workspace id of the park buffer
+  int split = 0;  // BluesteinLds / BluesteinCombine: workgroups per sequence of a split convolution
   PassArgs pa{};
   LeafArgs la{};
   StageArgs sa{};
@@ -52,6 +67,7 @@ struct Step {
   RealPassArgs ra{};
   MfmaArgs ma{};
   BluesteinLdsArgs ba{};
+  BluesteinCombineArgs ca{};
   int compute = 0;  // Mfma: kComputeBf16x3 / kComputeFp16x3
   LaunchShape shape{};
   int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)
@@ -59,8 +75,9 @@ struct Step {
   const void *func = nullptr;  // kernel handle, set by Executable::resolve_kernels()
 
   static constexpr size_t kArgBytes = 1024;
-  // The step's argument struct with its in/out pointers set; returns its size.
-  size_t pack_args(void *src, void *dst, unsigned char *buf) const;
+  // The step's argument struct with its in/out (and aux) pointers set;
+  // returns its size.
+  size_t pack_args(void *src, void *dst, unsigned char *buf, void *aux = nullptr) const;
   // The same launch for `batch` sequences of a 1D plan built for `built` (every
   // per-batch count of a 1D plan is proportional to the batch).
   Step rebatched(int64_t batch, int64_t built) const;
@@ -151,6 +168,8 @@ class Builder {
   const void *roots(int64_t count, int64_t n, int64_t mult);
   const void *chirp(int64_t n);
   const void *bhat_bluestein(int64_t n, int64_t M);
+  const void *bhat_bluestein_digit_reversed(int64_t n, int64_t M, const std::vector<int> &radices);
+  int cu_count();
   const void *rader_idx(int64_t p, int64_t g);
   const void *bhat_rader(int64_t p, int64_t g);
   const void *fp64_transform_then_cast(const std::string &key, const std::shared_ptr<DevBuf> &src64, int64_t len,
@@ -173,6 +192,7 @@ class Builder {
                       std::initializer_list<int> avoid_extra);
   void emit_rader(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  void emit_bluestein_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m, int split);
   void emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
                    int64_t R);

@@ -162,8 +162,26 @@ struct BluesteinLdsArgs {
   const void *tw;        // w_M^j, j < M
   const void *chirp;     // n entries
   const void *bhat;      // FFT_M(b) / M
+  void *park;            // two-halves kernels: n elements per sequence
   int32_t src_mode, dst_mode;
   int32_t conj_in, conj_out;
+  double scale;
+};
+
+// Last stage of a split LDS Bluestein convolution: W[e] = sum_h Z_h[e mod ml]
+// w_M^(h e), y[e] = conj(W[e]) chirp[e] for e < n (M = P ml).
+struct BluesteinCombineArgs {
+  const void *in;  // the P blocks Z_h of every sequence (P ml elements each)
+  void *out;
+  Geometry go;
+  int32_t ncol;
+  FastDiv fd_ncol;
+  int32_t nb;  // workgroups per sequence
+  FastDiv fd_nb;
+  int32_t n, ml;
+  const void *tw;     // w_M^j, j < M
+  const void *chirp;  // n entries
+  int32_t dst_mode, conj_out;
   double scale;
 };
 

@@ -51,10 +51,27 @@ const void *queue_kernel();
 
 // Bluestein kernels with the convolution in LDS (FP32): id for the
 // convolution's radix sequence (-1 if none), elements per thread, handle.
-int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl);
+// LDS-resident Bluestein instances: the radices of the part in LDS, and
+// whether the kernel runs the convolution as two halves (M = 2 x that part).
+struct BluesteinLdsShape {
+  std::vector<int> radices;
+  bool halves = false;
+  int64_t m = 0;  // convolution length
+};
+int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl, bool halves);
 int bluestein_lds_ept(int id);
-const void *bluestein_lds_kernel(int id, int variant);  // variant 1: prefetched base twiddles
-std::vector<std::vector<int>> bluestein_lds_radices();
+bool bluestein_lds_has_stockham(int id);
+// variant 0: Stockham stages; 1: Stockham, base twiddles prefetched; 2 / 3:
+// in-place stages with table / binary-powered twiddles (bhat in
+// digit-reversed order); instances without Stockham kernels ignore it
+const void *bluestein_lds_kernel(int id, int variant);
+int bluestein_lds_threads(int id, int variant);
+std::vector<BluesteinLdsShape> bluestein_lds_candidates(bool dbl);
+// Split form of a length-m convolution over `split` workgroups per sequence
+// (-1 when no instance), the radices of its LDS block, and the combine kernel.
+int bluestein_split_kernel_id(int64_t m, bool dbl, int split);
+std::vector<int> bluestein_lds_kernel_radices(int id);
+const void *bluestein_combine_kernel(bool dbl, int split);
 
 // Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).
 const void *mfma_kernel(int mode);

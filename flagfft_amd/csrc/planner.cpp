@@ -1,6 +1,8 @@
 // planner.cpp — decomposition choice (host only).
 #include "planner.hpp"
 
+#include "kernels.hpp"
+
 #include <algorithm>
 #include <sstream>
 #include <stdexcept>
@@ -237,24 +239,20 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   double best_cost = 0.0;
   bool have = false;
   const int64_t m0 = 2 * n - 1;
-  if (p == Prec::F32 && m0 > lmax && o.forced_passes.empty()) {
-    // whole convolution in LDS by a specialised instance: one launch
-    int64_t bm = 0;
-    std::vector<int> br;
-    for (const std::vector<int> &r : bluestein_lds_radices()) {
-      int64_t M = 1;
-      for (int x : r) M *= x;
-      if (M >= m0 && M <= 2 * m0 && (bm == 0 || M < bm)) {
-        bm = M;
-        br = r;
-      }
-    }
-    if (bm > 0) {
+  if (m0 > lmax && o.forced_passes.empty()) {
+    // whole convolution in LDS by a specialised instance: one launch (the
+    // two-halves instances hold M / 2 at a time)
+    const BluesteinLdsShape *pick = nullptr;
+    const std::vector<BluesteinLdsShape> cands = bluestein_lds_candidates(p == Prec::F64);
+    for (const BluesteinLdsShape &sh : cands)
+      if (sh.m >= m0 && sh.m <= 2 * m0 && (pick == nullptr || sh.m < pick->m)) pick = &sh;
+    if (pick != nullptr) {
       c.kind = CoreKind::Bluestein;
-      c.m = bm;
-      c.radices = br;
+      c.m = pick->m;
+      c.radices = pick->radices;
       c.conv.reset();
       c.lds_conv = true;
+      c.lds_halves = pick->halves;
       return c;
     }
   }
@@ -317,9 +315,14 @@ std::string Core::describe(int indent) const {
       break;
     case CoreKind::Bluestein:
       o << pad << "Bluestein(n=" << n << ", conv_length=" << m << ", fused=" << (conv ? 0 : 1)
-        << (lds_conv ? ", lds_resident=1" : "") << ")\n";
-      if (conv) o << conv->describe(indent + 2);
-      else o << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
+        << (lds_conv ? ", lds_resident=1" : "") << (lds_halves ? ", halves=1" : "") << ")\n";
+      if (conv)
+        o << conv->describe(indent + 2);
+      else if (lds_halves)
+        o << pad << "  conv: Leaf(n=" << m << ", radices=[2," << join(radices, ",") << "], in LDS: two halves of "
+          << m / 2 << ")";
+      else
+        o << pad << "  conv: Leaf(n=" << m << ", radices=[" << join(radices, ",") << "])";
       break;
   }
   return o.str();

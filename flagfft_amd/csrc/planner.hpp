@@ -52,6 +52,7 @@ struct Core {
   std::shared_ptr<Core> conv;                // Bluestein: multi-pass core of M (null when fused)
   int compute = 0;                           // MatrixCore: 1 = BF16 hi/lo, 2 = FP16 hi/lo operands
   bool lds_conv = false;                     // Bluestein: convolution in LDS by a specialised kernel
+  bool lds_halves = false;                   // lds_conv as two halves of M / 2 (radices cover M / 2)
   std::string describe(int indent = 0) const;
 };
 
@@ -94,8 +95,6 @@ inline int lds_pad(int e) { return e + (e >> 5); }   // must match the kernels' 
 // Catalogue of the compile-time specialised super-stage kernels (leaf.hip):
 // id of the kernel for this radix sequence, -1 when none is compiled.
 int stage_kernel_id(const std::vector<int> &radices);
-// Radix sequences of the LDS-resident Bluestein instances (FP32).
-std::vector<std::vector<int>> bluestein_lds_radices();
 // Same for the specialised single-pass kernels.
 int leaf_kernel_id(const std::vector<int> &radices);
 // True when every super-radix of the split has a specialised kernel.

@@ -111,5 +111,113 @@ __device__ __forceinline__ void prefetch_base_twiddles(V2<T> *tw_reg, const GlbC
   });
 }
 
+// w^j for a compile-time j from the binary powers p[b] = w^(2^b): at most
+// popcount(j) - 1 products, so a few roundings instead of one table load per j.
+template <typename T, int J>
+__device__ __forceinline__ V2<T> tw_power(const V2<T> *p) {
+  V2<T> r{1, 0};
+  bool first = true;
+  sfor<4>([&](auto B) __attribute__((always_inline)) {
+    constexpr int b = decltype(B)::value;
+    if constexpr ((J >> b) & 1) {
+      r = first ? p[b] : cmul(r, p[b]);
+      first = false;
+    }
+  });
+  return r;
+}
+
+// Radix stages of one R-point transform spread over TPU lanes (lane t).
+// LDS index e is padded by one slot every 2^PS elements.
+// Stage 0 takes element e from load0(e); inner stages exchange through LDS
+// (buf); the last stage hands natural-order element idx to sink(idx, v).
+//   kLdsIn   load0 reads LDS too, so the first LDS write waits for every lane
+//   sink_lds the sink writes LDS: wait for every lane's last-stage reads first
+// Every lane of the workgroup must call this (it contains barriers).
+// kTwPow: inner-stage twiddles w^(j k) by binary powers of the butterfly's
+// base twiddle w^k, which twr holds (prefetch_base_twiddles), instead of R - 1
+// table loads; for register-bound kernels.
+template <typename T, int E, int TPU, typename RS, bool kLdsIn, int PS = 5, bool kTwPow = false,
+          bool kPreBase = false, typename Load, typename Sink>
+__device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr, const GlbC<T> *tws, bool sink_lds,
+                                           Load load0, Sink sink) {
+  auto lpad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
+  constexpr int R = RS::kN;
+  constexpr bool kPre = !kTwPow && RS::template prefetch<T, E>();
+  sfor<RS::kS>([&](auto S) __attribute__((always_inline)) {
+    constexpr int s = decltype(S)::value;
+    constexpr int RR = RS::kR[s];
+    constexpr int q = RS::prior(s);
+    constexpr int NB = R / RR;
+    constexpr int MB = RS::template butterflies<E>(s);
+    V2<T> x[MB * RR];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const int b = t + m * TPU;
+      if (b >= NB) continue;
+#pragma unroll
+      for (int j = 0; j < RR; ++j) {
+        const int e = b + j * NB;
+        if constexpr (s == 0)
+          x[m * RR + j] = load0(e);
+        else
+          x[m * RR + j] = ldc(buf + lpad(e));
+      }
+      if constexpr (q > 1) {
+        if constexpr (kPre) {
+          constexpr int off = RS::template tw_offset<E>(s);
+#pragma unroll
+          for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], twr[off + m * (RR - 1) + (j - 1)]);
+        } else if constexpr (kTwPow) {
+          V2<T> pw[4];
+          if constexpr (kPreBase) {
+            pw[0] = twr[RS::template pow_offset<E>(s) + m];  // prefetched base twiddle w^k
+          } else {
+            constexpr int ts = R / (q * RR);
+            pw[0] = ldt(tws, static_cast<uint32_t>(b % q) * ts);
+          }
+          pw[1] = cmul(pw[0], pw[0]);
+          pw[2] = cmul(pw[1], pw[1]);
+          pw[3] = cmul(pw[2], pw[2]);
+          sfor<RR>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j > 0) x[m * RR + j] = cmul(x[m * RR + j], tw_power<T, j>(pw));
+          });
+        } else {
+          constexpr int ts = R / (q * RR);
+          const uint32_t kk = static_cast<uint32_t>(b % q);
+#pragma unroll
+          for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], ldt(tws, j * kk * ts));
+        }
+      }
+      dft<RR>(&x[m * RR]);
+    }
+    if constexpr (s + 1 < RS::kS) {
+      if constexpr (s > 0 || kLdsIn) __syncthreads();
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const int b = t + m * TPU;
+        if (b >= NB) continue;
+        const int kk = (q > 1) ? b % q : 0;
+        const int base = (b - kk) * RR + kk;
+#pragma unroll
+        for (int j = 0; j < RR; ++j) stc(buf + lpad(base + j * q), x[m * RR + j]);
+      }
+      __syncthreads();
+    } else {
+      if constexpr (RS::kS > 1 || kLdsIn) {
+        if (sink_lds) __syncthreads();
+      }
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const int b = t + m * TPU;
+        if (b >= NB) continue;
+#pragma unroll
+        for (int j = 0; j < RR; ++j) sink(b + j * q, x[m * RR + j]);
+      }
+    }
+  });
+}
+
 }  // namespace dev
 }  // namespace flagfft

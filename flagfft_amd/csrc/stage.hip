@@ -14,158 +14,6 @@
 namespace flagfft {
 namespace dev {
 
-// Radix stages of one R-point transform spread over TPU lanes (lane t).
-// LDS index e is padded by one slot every 2^PS elements.
-// Stage 0 takes element e from load0(e); inner stages exchange through LDS
-// (buf); the last stage hands natural-order element idx to sink(idx, v).
-//   kLdsIn   load0 reads LDS too, so the first LDS write waits for every lane
-//   sink_lds the sink writes LDS: wait for every lane's last-stage reads first
-// Every lane of the workgroup must call this (it contains barriers).
-// w^j for a compile-time j from the binary powers p[b] = w^(2^b): at most
-// popcount(j) - 1 products, so a few roundings instead of one table load per j.
-template <typename T, int J>
-__device__ __forceinline__ V2<T> tw_power(const V2<T> *p) {
-  V2<T> r{1, 0};
-  bool first = true;
-  sfor<4>([&](auto B) __attribute__((always_inline)) {
-    constexpr int b = decltype(B)::value;
-    if constexpr ((J >> b) & 1) {
-      r = first ? p[b] : cmul(r, p[b]);
-      first = false;
-    }
-  });
-  return r;
-}
-
-// kTwPow: inner-stage twiddles w^(j k) by binary powers of the butterfly's
-// base twiddle w^k, which twr holds (prefetch_base_twiddles), instead of R - 1
-// table loads; for register-bound kernels.
-template <typename T, int E, int TPU, typename RS, bool kLdsIn, int PS = 5, bool kTwPow = false,
-          bool kPreBase = false, typename Load, typename Sink>
-__device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr, const GlbC<T> *tws, bool sink_lds,
-                                           Load load0, Sink sink) {
-  auto lpad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
-  constexpr int R = RS::kN;
-  constexpr bool kPre = !kTwPow && RS::template prefetch<T, E>();
-  sfor<RS::kS>([&](auto S) __attribute__((always_inline)) {
-    constexpr int s = decltype(S)::value;
-    constexpr int RR = RS::kR[s];
-    constexpr int q = RS::prior(s);
-    constexpr int NB = R / RR;
-    constexpr int MB = RS::template butterflies<E>(s);
-    V2<T> x[MB * RR];
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      const int b = t + m * TPU;
-      if (b >= NB) continue;
-#pragma unroll
-      for (int j = 0; j < RR; ++j) {
-        const int e = b + j * NB;
-        if constexpr (s == 0)
-          x[m * RR + j] = load0(e);
-        else
-          x[m * RR + j] = ldc(buf + lpad(e));
-      }
-      if constexpr (q > 1) {
-        if constexpr (kPre) {
-          constexpr int off = RS::template tw_offset<E>(s);
-#pragma unroll
-          for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], twr[off + m * (RR - 1) + (j - 1)]);
-        } else if constexpr (kTwPow) {
-          V2<T> pw[4];
-          if constexpr (kPreBase) {
-            pw[0] = twr[RS::template pow_offset<E>(s) + m];  // prefetched base twiddle w^k
-          } else {
-            constexpr int ts = R / (q * RR);
-            pw[0] = ldt(tws, static_cast<uint32_t>(b % q) * ts);
-          }
-          pw[1] = cmul(pw[0], pw[0]);
-          pw[2] = cmul(pw[1], pw[1]);
-          pw[3] = cmul(pw[2], pw[2]);
-          sfor<RR>([&](auto J) __attribute__((always_inline)) {
-            constexpr int j = decltype(J)::value;
-            if constexpr (j > 0) x[m * RR + j] = cmul(x[m * RR + j], tw_power<T, j>(pw));
-          });
-        } else {
-          constexpr int ts = R / (q * RR);
-          const uint32_t kk = static_cast<uint32_t>(b % q);
-#pragma unroll
-          for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], ldt(tws, j * kk * ts));
-        }
-      }
-      dft<RR>(&x[m * RR]);
-    }
-    if constexpr (s + 1 < RS::kS) {
-      if constexpr (s > 0 || kLdsIn) __syncthreads();
-#pragma unroll
-      for (int m = 0; m < MB; ++m) {
-        const int b = t + m * TPU;
-        if (b >= NB) continue;
-        const int kk = (q > 1) ? b % q : 0;
-        const int base = (b - kk) * RR + kk;
-#pragma unroll
-        for (int j = 0; j < RR; ++j) stc(buf + lpad(base + j * q), x[m * RR + j]);
-      }
-      __syncthreads();
-    } else {
-      if constexpr (RS::kS > 1 || kLdsIn) {
-        if (sink_lds) __syncthreads();
-      }
-#pragma unroll
-      for (int m = 0; m < MB; ++m) {
-        const int b = t + m * TPU;
-        if (b >= NB) continue;
-#pragma unroll
-        for (int j = 0; j < RR; ++j) sink(b + j * q, x[m * RR + j]);
-      }
-    }
-  });
-}
-
-// Bluestein with the whole length-M convolution in LDS (one launch, one
-// sequence per workgroup), for M above the default single-pass limit:
-//   a = chirp * x (zero past n), A = FFT_M(a) -> LDS, W = FFT_M(conj(A * bhat)),
-//   y[k] = conj(W[k]) * chirp[k], k < n           (src/exec/raw_nodes.cpp:153-204)
-// One pad slot every 64 elements so M = 20160 fits the 160 KiB of LDS; up to
-// 1024 lanes per workgroup keep the per-lane element count (and registers) low.
-template <typename T, int E, bool kPreBase, int... Rs>
-__global__ __launch_bounds__(1024) void k_bluestein_lds(const BluesteinLdsArgs a) {
-  using RS = RadixSeq<Rs...>;
-  constexpr int M = RS::kN;
-  constexpr int TPU = (M + E - 1) / E;
-  constexpr int PS = 6;
-  LdsC<T> *buf = lds_base<T>();
-  const int t = threadIdx.x;
-  const uint32_t q = blockIdx.x;
-  const uint32_t b = fdiv(q, a.fd_ncol);
-  const uint32_t c = q - b * static_cast<uint32_t>(a.ncol);
-  const int64_t ibase = static_cast<int64_t>(b) * a.gi.bs + static_cast<int64_t>(c) * a.gi.cs;
-  const int64_t obase = static_cast<int64_t>(b) * a.go.bs + static_cast<int64_t>(c) * a.go.cs;
-  const GlbC<T> *tws = gc<T>(a.tw);
-  const GlbC<T> *bhat = gc<T>(a.bhat);
-  const T scale = static_cast<T>(a.scale);
-  // kPreBase: one base twiddle per inner-stage butterfly loaded up front and
-  // shared by both transforms (more registers), else one load per stage
-  V2<T> twr[kPreBase ? RS::template pow_total<E>() : 1];
-  if constexpr (kPreBase) prefetch_base_twiddles<T, E, TPU, RS>(twr, tws, t);
-  auto pad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
-  auto to_lds = [&](int idx, V2<T> v) __attribute__((always_inline)) { stc(buf + pad(idx), v); };
-  run_stages<T, E, TPU, RS, false, PS, true, kPreBase>(
-      buf, t, twr, tws, true,
-      [&](int e) __attribute__((always_inline)) {
-        return src_elem<T>(a.in, ibase, a.gi.es, a.src_mode | kSrcChirpFlag, a.n, a.chirp, a.conj_in, e);
-      },
-      to_lds);
-  __syncthreads();
-  run_stages<T, E, TPU, RS, true, PS, true, kPreBase>(
-      buf, t, twr, tws, false,
-      [&](int e) __attribute__((always_inline)) { return cconj(cmul(ldc(buf + pad(e)), ldg(bhat + e))); },
-      [&](int idx, V2<T> v) __attribute__((always_inline)) {
-        dst_elem<T>(a.out, obase, a.go.es, a.dst_mode | kDstChirpFlag, a.n, a.chirp, nullptr, a.conj_out, scale,
-                    idx, v);
-      });
-}
-
 // Super-stage modes.
 //   kStagePass         one pass of a chain: unit u = b * tcnt + i of sequence b
 //                      reads x[i + e tcnt] (e < R), multiplies by
@@ -332,47 +180,7 @@ const StageEntry kStageTable[] = {
     FLAGFFT_STAGE3(8, 32, 4, 32, 16, 16, 8),    // 2048
 };
 
-struct BluesteinLdsEntry {
-  int nrad;
-  int rad[4];
-  int e32;
-  const void *f32;      // base twiddles loaded per stage
-  const void *f32_pre;  // base twiddles prefetched at kernel start
-};
-
-// FP32 only: the FP64 convolution of these lengths does not fit in LDS.
-const BluesteinLdsEntry kBluesteinLdsTable[] = {
-    {4, {16, 16, 16, 4}, 16, (const void *)dev::k_bluestein_lds<float, 16, false, 16, 16, 16, 4>,
-     (const void *)dev::k_bluestein_lds<float, 16, true, 16, 16, 16, 4>},  // 16384
-    {4, {14, 12, 12, 10}, 20, (const void *)dev::k_bluestein_lds<float, 20, false, 14, 12, 12, 10>,
-     (const void *)dev::k_bluestein_lds<float, 20, true, 14, 12, 12, 10>},  // 20160
-};
-
 }  // namespace
-
-int bluestein_lds_kernel_id(const std::vector<int> &radices, bool dbl) {
-  if (dbl) return -1;
-  const int n = static_cast<int>(sizeof(kBluesteinLdsTable) / sizeof(kBluesteinLdsTable[0]));
-  for (int i = 0; i < n; ++i) {
-    const BluesteinLdsEntry &e = kBluesteinLdsTable[i];
-    if (static_cast<int>(radices.size()) != e.nrad) continue;
-    bool same = true;
-    for (int j = 0; j < e.nrad; ++j) same = same && radices[static_cast<size_t>(j)] == e.rad[j];
-    if (same) return i;
-  }
-  return -1;
-}
-
-int bluestein_lds_ept(int id) { return kBluesteinLdsTable[id].e32; }
-const void *bluestein_lds_kernel(int id, int variant) {
-  return variant == 1 ? kBluesteinLdsTable[id].f32_pre : kBluesteinLdsTable[id].f32;
-}
-// Lengths with an instance, ascending (the planner's candidates).
-std::vector<std::vector<int>> bluestein_lds_radices() {
-  std::vector<std::vector<int>> out;
-  for (const BluesteinLdsEntry &e : kBluesteinLdsTable) out.emplace_back(e.rad, e.rad + e.nrad);
-  return out;
-}
 
 int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units) {
   const int n = static_cast<int>(sizeof(kStageTable) / sizeof(kStageTable[0]));
