@@ -48,10 +48,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
   GlbC<T> *dst = gc<T>(a.out) + (active ? seq * a.odist : 0);
   const T scale = static_cast<T>(a.scale);
   const GlbC<T> *wn = gc<T>(a.aux);
-  constexpr bool kPre = RS::template prefetch<T, E>();
   const GlbC<T> *tw = gc<T>(a.tw);
-  V2<T> twr[kPre ? RS::template tw_total<E>() : 1];
-  if constexpr (kPre) prefetch_twiddles<T, E, TPU, RS>(twr, tw, t);
   if constexpr (Mode == kLeafC2R) {
     // fused pre-process into LDS (conjugated: the inverse runs as a forward transform)
     for (int k = t; k <= N / 2; k += TPU) {
@@ -101,16 +98,10 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
           x[m * RR + j] = v;
         }
         if constexpr (q > 1) {
-          if constexpr (kPre) {
-            constexpr int off = RS::template tw_offset<E>(s);
-#pragma unroll
-            for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], twr[off + m * (RR - 1) + (j - 1)]);
-          } else {
-            constexpr int ts = N / (q * RR);
-            const int k = b % q;
-#pragma unroll
-            for (int j = 1; j < RR; ++j) x[m * RR + j] = cmul(x[m * RR + j], ldg(tw + j * k * ts));
-          }
+          // w^(j k) by binary powers of the butterfly's base twiddle: one
+          // table load per butterfly instead of RR - 1
+          constexpr int ts = N / (q * RR);
+          twiddle_powers<T, RR>(&x[m * RR], ldg(tw + (b % q) * ts));
         }
         dft<RR>(&x[m * RR]);
       }

@@ -127,6 +127,20 @@ __device__ __forceinline__ V2<T> tw_power(const V2<T> *p) {
   return r;
 }
 
+// x[j] *= w^j, j = 1..R-1, with w^j formed by binary powers of w.
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_powers(V2<T> *x, V2<T> w) {
+  V2<T> pw[4];
+  pw[0] = w;
+  pw[1] = cmul(pw[0], pw[0]);
+  pw[2] = cmul(pw[1], pw[1]);
+  pw[3] = cmul(pw[2], pw[2]);
+  sfor<R>([&](auto J) __attribute__((always_inline)) {
+    constexpr int jj = decltype(J)::value;
+    if constexpr (jj > 0) x[jj] = cmul(x[jj], tw_power<T, jj>(pw));
+  });
+}
+
 // Radix stages of one R-point transform spread over TPU lanes (lane t).
 // LDS index e is padded by one slot every 2^PS elements.
 // Stage 0 takes element e from load0(e); inner stages exchange through LDS
@@ -137,10 +151,18 @@ __device__ __forceinline__ V2<T> tw_power(const V2<T> *p) {
 // kTwPow: inner-stage twiddles w^(j k) by binary powers of the butterfly's
 // base twiddle w^k, which twr holds (prefetch_base_twiddles), instead of R - 1
 // table loads; for register-bound kernels.
+// No extra work on the first stage's inputs.
+struct NoPre0 {
+  template <typename C>
+  __device__ __forceinline__ void operator()(int, C *) const {}
+};
+
+// pre0(b, x) runs on the r inputs of first-stage butterfly b after load0
+// (the inter-pass twiddles of a super-stage).
 template <typename T, int E, int TPU, typename RS, bool kLdsIn, int PS = 5, bool kTwPow = false,
-          bool kPreBase = false, typename Load, typename Sink>
+          bool kPreBase = false, typename Load, typename Sink, typename Pre0 = NoPre0>
 __device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr, const GlbC<T> *tws, bool sink_lds,
-                                           Load load0, Sink sink) {
+                                           Load load0, Sink sink, Pre0 pre0 = Pre0{}) {
   auto lpad = [](int e) __attribute__((always_inline)) { return e + (e >> PS); };
   constexpr int R = RS::kN;
   constexpr bool kPre = !kTwPow && RS::template prefetch<T, E>();
@@ -163,6 +185,7 @@ __device__ __forceinline__ void run_stages(LdsC<T> *buf, int t, const V2<T> *twr
         else
           x[m * RR + j] = ldc(buf + lpad(e));
       }
+      if constexpr (s == 0) pre0(b, &x[m * RR]);
       if constexpr (q > 1) {
         if constexpr (kPre) {
           constexpr int off = RS::template tw_offset<E>(s);

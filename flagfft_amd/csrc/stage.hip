@@ -63,9 +63,31 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage(const StageArgs a) {
   const T scale = static_cast<T>(a.scale);
   const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
   LdsC<T> *buf = lds + sl * LS;
-  constexpr bool kPre = RS::template prefetch<T, E>();
-  V2<T> twr[kPre ? RS::template tw_total<E>() : 1];
-  if constexpr (kPre) prefetch_twiddles<T, E, TPU, RS>(twr, tws, t);
+  // inner-stage twiddles by binary powers of one base twiddle per butterfly
+  V2<T> twr[1];
+  // inter-pass twiddle of element e = b + j NB of first-stage butterfly b:
+  // w^(e kmul) = w^(b kmul) (w^(NB kmul))^j, two-level table lookups of the
+  // base and the step, powers of the step by squaring
+  constexpr int NB0 = R / RS::kR[0];
+  auto tw2 = [&](uint32_t mm) __attribute__((always_inline)) {
+    return cmul(ldg(tw_hi + (mm >> a.tw_bits)), ldg(tw_lo + (mm & tw_mask)));
+  };
+  const V2<T> step = first ? V2<T>{1, 0} : tw2(static_cast<uint32_t>(NB0) * kmul);
+  auto pre0 = [&](int b, V2<T> *x) __attribute__((always_inline)) {
+    if (first) return;
+    constexpr int R0 = RS::kR[0];
+    const V2<T> base = tw2(static_cast<uint32_t>(b) * kmul);
+    V2<T> pw[4];
+    pw[0] = step;
+    pw[1] = cmul(pw[0], pw[0]);
+    pw[2] = cmul(pw[1], pw[1]);
+    pw[3] = cmul(pw[2], pw[2]);
+    x[0] = cmul(x[0], base);
+    sfor<R0>([&](auto J) __attribute__((always_inline)) {
+      constexpr int jj = decltype(J)::value;
+      if constexpr (jj > 0) x[jj] = cmul(x[jj], cmul(base, tw_power<T, jj>(pw)));
+    });
+  };
 
   auto load_global = [&](int e) __attribute__((always_inline)) {
     const int64_t elog = static_cast<int64_t>(i) + e * tcnt;
@@ -75,10 +97,6 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage(const StageArgs a) {
       if (a.conj_in) v.y = -v.y;
       if (chirp_in) v = cmul(v, ldg(chirp + elog));
     }
-    if (!first) {
-      const uint32_t mm = static_cast<uint32_t>(e) * kmul;
-      v = cmul(v, cmul(ldg(tw_hi + (mm >> a.tw_bits)), ldg(tw_lo + (mm & tw_mask))));
-    }
     return v;
   };
   auto to_lds = [&](int idx, V2<T> v) __attribute__((always_inline)) { stc(buf + lpad(idx), v); };
@@ -86,26 +104,27 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage(const StageArgs a) {
   bool block_store = a.block_store != 0;
   if constexpr (Mode == kStageConvolution) {
     block_store = true;
-    run_stages<T, E, TPU, RS, false>(buf, t, twr, tws, true, load_global, to_lds);
+    run_stages<T, E, TPU, RS, false, 5, true>(buf, t, twr, tws, true, load_global, to_lds, pre0);
     __syncthreads();
     const GlbC<T> *bhat = gc<T>(a.bhat);
-    run_stages<T, E, TPU, RS, true>(
+    run_stages<T, E, TPU, RS, true, 5, true>(
         buf, t, twr, tws, true,
         [&](int e) __attribute__((always_inline)) {
           return cconj(cmul(ldc(buf + lpad(e)), ldg(bhat + (i + static_cast<int64_t>(e) * a.p))));
         },
         to_lds);
   } else if (block_store) {
-    run_stages<T, E, TPU, RS, false>(buf, t, twr, tws, true, load_global, to_lds);
+    run_stages<T, E, TPU, RS, false, 5, true>(buf, t, twr, tws, true, load_global, to_lds, pre0);
   } else {
-    run_stages<T, E, TPU, RS, false>(buf, t, twr, tws, false, load_global,
-                                     [&](int idx, V2<T> v) __attribute__((always_inline)) {
-                                       if (!active) return;
-                                       const int64_t oidx = (i - k) * static_cast<int64_t>(R) + k +
-                                                            static_cast<int64_t>(idx) * a.p;
-                                       dst_elem<T>(a.out, static_cast<int64_t>(bq) * a.bs_out, 1, a.dst_mode,
-                                                   a.nlog, a.chirp, a.bhat, a.conj_out, scale, oidx, v);
-                                     });
+    run_stages<T, E, TPU, RS, false, 5, true>(
+        buf, t, twr, tws, false, load_global,
+        [&](int idx, V2<T> v) __attribute__((always_inline)) {
+          if (!active) return;
+          const int64_t oidx = (i - k) * static_cast<int64_t>(R) + k + static_cast<int64_t>(idx) * a.p;
+          dst_elem<T>(a.out, static_cast<int64_t>(bq) * a.bs_out, 1, a.dst_mode, a.nlog, a.chirp, a.bhat,
+                      a.conj_out, scale, oidx, v);
+        },
+        pre0);
   }
   if (block_store) {
     // the workgroup's C units own one contiguous block of C * R output elements
