@@ -84,7 +84,12 @@ def candidates(type_, n: int, batch: int, describe: str):
     cands = [{}]
     dbl = type_ in DOUBLE
     core_n = n // 2 if type_ in (T.R2C, T.D2Z, T.C2R, T.Z2D) and n % 2 == 0 else n
-    if "Leaf(" in describe:
+    if "lds_resident=1" in describe:
+        # LDS Bluestein forms: 1-4 kernel variants (whole convolution per
+        # workgroup), 5 whole convolution, 6 / 7 split over 2 / 4 workgroups
+        for units in range(1, 8):
+            cands.append({"units": units})
+    elif "Leaf(" in describe:
         rad = _radices(core_n) or []
         orders = sorted(set(itertools.permutations(rad)), reverse=True)[:6]
         for o in orders:

@@ -19,8 +19,10 @@
 // step 1 is reused as step 2's B operand without any data movement: chunk q
 // of the K loop pairs lane group h, element v with n2 = 16q + 8(v/4) + 4h + v%4,
 // which is accumulator register 8q + v, and step 2's F operand is permuted the
-// same way.  Loads (x[32 n1 + c]) and stores (X[c + 32 k2]) are 256-byte row
-// segments per half wave.
+// same way.  Step 1 sums over n1 = 16q + 2v + h and step 2's output rows are
+// permuted (k2 = pi(row), see out_row) so that the two half waves load
+// (x[32 n1 + c]) and store (X[c + 32 k2]) adjacent rows: one 512-byte run per
+// instruction.
 //
 // FP16 has a narrow exponent range, so each transform is scaled by a power of
 // two (exact) that brings max |x| below 1 before the split, and scaled back on
@@ -86,11 +88,15 @@ __device__ __forceinline__ Split8<F16> neg(const Split8<F16> &s) {
   return Split8<F16>{-s.hi, -s.lo};
 }
 
+// Output row permutation of step 2: accumulator row i = 8a + 4h + b holds
+// k2 = 8a + 2b + h, so the two half waves store adjacent 256-byte rows.
+__device__ __forceinline__ int out_row(int i) { return 8 * (i >> 3) + 2 * (i & 3) + ((i >> 2) & 1); }
+
 // w_1024^e from the FP32 table (e reduced mod 1024).
 __device__ __forceinline__ V2<float> w1024(const GlbC<float> *tw, int e) { return ldg(tw + (e & 1023)); }
 
 template <bool F16>
-__global__ __launch_bounds__(256) void k_mfma1024(const MfmaArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma1024(const MfmaArgs a) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 31;
   const int h = lane >> 5;
@@ -99,15 +105,16 @@ __global__ __launch_bounds__(256) void k_mfma1024(const MfmaArgs a) {
   const GlbC<float> *tw = gc<float>(a.tw);
 
   // per-lane constant operands (identical for every transform this wave handles)
-  Split8<F16> fb_r[2], fb_i[2];  // step 1 B operand: F[16q + 8h + v][c]
-  Split8<F16> fa_r[2], fa_i[2];  // step 2 A operand: F[c][16q + 8(v/4) + 4h + v%4]
+  Split8<F16> fb_r[2], fb_i[2];  // step 1 B operand: F[16q + 2v + h][c]
+  Split8<F16> fa_r[2], fa_i[2];  // step 2 A operand: F[pi(c)][16q + 8(v/4) + 4h + v%4]
   V2<float> twid[16];            // w_1024^(n2 c), n2 = 8(r/4) + 4h + r%4
+  const int pc = out_row(c);
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      const V2<float> f1 = w1024(tw, 32 * (16 * q + 8 * h + v) * c);
-      const V2<float> f2 = w1024(tw, 32 * c * (16 * q + 8 * (v >> 2) + 4 * h + (v & 3)));
+      const V2<float> f1 = w1024(tw, 32 * (16 * q + 2 * v + h) * c);
+      const V2<float> f2 = w1024(tw, 32 * pc * (16 * q + 8 * (v >> 2) + 4 * h + (v & 3)));
       split_into<F16>(fb_r[q], v, f1.x);
       split_into<F16>(fb_i[q], v, f1.y);
       split_into<F16>(fa_r[q], v, f2.x);
@@ -121,14 +128,25 @@ __global__ __launch_bounds__(256) void k_mfma1024(const MfmaArgs a) {
   const float cout = a.conj_out ? -1.f : 1.f;
   const float user_scale = static_cast<float>(a.scale);
 
+  // software pipeline: the next transform's loads are in flight while this
+  // one runs through the matrix cores (two waves per SIMD fit the registers)
+  auto load = [&](int64_t sq, V2<float> (&x)[2][8]) __attribute__((always_inline)) {
+    const GlbC<float> *src = gc<float>(a.in) + sq * a.idist + c;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) x[q][v] = ldg(src + 32 * (16 * q + 2 * v + h));
+  };
+  V2<float> xn[2][8];
+  if (wave < a.nseq) load(wave, xn);
   for (int64_t seq = wave; seq < a.nseq; seq += nwaves) {
-    const GlbC<float> *src = gc<float>(a.in) + seq * a.idist + c;
     GlbC<float> *dst = gc<float>(a.out) + seq * a.odist + c;
     V2<float> x[2][8];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int v = 0; v < 8; ++v) x[q][v] = ldg(src + 32 * (16 * q + 8 * h + v));
+      for (int v = 0; v < 8; ++v) x[q][v] = xn[q][v];
+    if (seq + nwaves < a.nseq) load(seq + nwaves, xn);
 
     // exact power-of-two range scaling (FP16 only)
     float pre = 1.f, post = 1.f;
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(256) void k_mfma1024(const MfmaArgs a) {
     const float so = post * user_scale;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int k2 = 8 * (r >> 2) + 4 * h + (r & 3);
+      const int k2 = 8 * (r >> 2) + 2 * (r & 3) + h;  // pi(8(r/4) + 4h + r%4)
       stg(dst + 32 * k2, V2<float>{cr[r] * so, cout * ci[r] * so});
     }
   }
