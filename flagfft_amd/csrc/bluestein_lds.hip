@@ -349,12 +349,24 @@ __device__ __forceinline__ void bluestein_split_body(const BluesteinLdsArgs &a) 
   constexpr int ML = RS::kN;
   constexpr int M = ML * P;
   static_assert(P == 2 || P == 4, "split into 2 or 4 blocks");
-  const uint32_t q = blockIdx.x / P;
-  const int h = static_cast<int>(blockIdx.x % P);
+  // workgroups go round-robin over the 8 XCDs: keep the P blocks of a
+  // sequence on one XCD (its L2 serves their repeated input reads) when the
+  // sequence count allows
+  uint32_t q, slot;
+  const uint32_t nseq = gridDim.x / P;
+  if ((nseq & 7u) == 0) {
+    const uint32_t xcd = blockIdx.x & 7u, local = blockIdx.x >> 3;
+    q = (local / P) * 8u + xcd;
+    slot = q * P + local % P;
+  } else {
+    q = blockIdx.x / P;
+    slot = blockIdx.x;
+  }
+  const int h = static_cast<int>(slot % P);
   const BlueIo<T, kFast> io(a, q);
   const int n = io.n;
   const GlbC<T> *tws = gc<T>(a.tw);
-  GlbC<T> *park = gc<T>(a.park) + static_cast<int64_t>(blockIdx.x) * ML;
+  GlbC<T> *park = gc<T>(a.park) + static_cast<int64_t>(slot) * ML;
   conv_lds<T, TPU, false, P, RS::kR[0], RS::kR[0], RS>(
       lds_base<T>(), threadIdx.x, tws, gc<T>(a.bhat) + h * ML,
       [&](int e) __attribute__((always_inline)) {
@@ -384,9 +396,19 @@ __global__ __launch_bounds__(TPU) void k_bluestein_split(const BluesteinLdsArgs 
 // (then conjugated / scaled and stored in the destination layout), e < n.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_bluestein_combine(const BluesteinCombineArgs a) {
-  const uint32_t x = blockIdx.x;
-  const uint32_t q = fdiv(x, a.fd_nb);
-  const int e = static_cast<int>(x - q * static_cast<uint32_t>(a.nb)) * 256 + static_cast<int>(threadIdx.x);
+  // same XCD as the split kernel's blocks of sequence q (see there)
+  uint32_t q, j;
+  const uint32_t nseq = gridDim.x / static_cast<uint32_t>(a.nb);
+  if ((nseq & 7u) == 0) {
+    const uint32_t local = blockIdx.x >> 3;
+    const uint32_t ql = fdiv(local, a.fd_nb);
+    q = ql * 8u + (blockIdx.x & 7u);
+    j = local - ql * static_cast<uint32_t>(a.nb);
+  } else {
+    q = fdiv(blockIdx.x, a.fd_nb);
+    j = blockIdx.x - q * static_cast<uint32_t>(a.nb);
+  }
+  const int e = static_cast<int>(j) * 256 + static_cast<int>(threadIdx.x);
   if (e >= a.n) return;
   const int ml = a.ml;
   const int m = P * ml;
