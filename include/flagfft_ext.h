@@ -49,7 +49,12 @@ const char *flagfftxStepLabel(flagfftHandle plan, int direction, int step);
 typedef struct flagfftxTuning_t {
   int leaf_max;       /* longest single-pass LDS transform */
   int pass_max;       /* longest super-radix of a multi-pass chain */
-  int units;          /* transforms (or butterflies) per workgroup */
+  int units;          /* transforms (or butterflies) per workgroup; for an
+                         LDS-resident Bluestein core the form instead: 1 / 2
+                         Stockham stages (prefetched / per-stage twiddles),
+                         3 / 4 in-place stages (table / binary-powered
+                         twiddles), 5 one workgroup per sequence, 6 / 7 the
+                         convolution split over 2 / 4 workgroups */
   int ept;            /* elements per thread: 16 or 32 (fp32), 8 or 16 (fp64) */
   int chunk_bytes;    /* contiguous bytes per row segment in strided passes */
   int npasses;        /* explicit super-radix split, `npasses` entries of `passes` */
