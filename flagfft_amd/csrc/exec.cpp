@@ -110,6 +110,7 @@ size_t Step::pack_args(void *src, void *dst, unsigned char *buf, void *aux) cons
       return put(b);
     }
     case BluesteinCombine: return put(ca);
+    case LeafSplit: return put(lsa);
     case Rader:
     case Bluestein:
     case R2CEven:
@@ -149,6 +150,10 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       break;
     case BluesteinLds:
     case BluesteinCombine:  // workgroups per sequence are fixed
+      s.shape.grid = dim3(static_cast<unsigned>(scale(static_cast<int64_t>(shape.grid.x))));
+      break;
+    case LeafSplit:
+      s.lsa.nseq = scale(lsa.nseq);
       s.shape.grid = dim3(static_cast<unsigned>(scale(static_cast<int64_t>(shape.grid.x))));
       break;
     case Rader:
@@ -196,6 +201,7 @@ void Executable::resolve_kernels() {
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
+      case Step::LeafSplit: s.func = leaf_split_kernel(s.leaf_id, s.dbl); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
       case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
@@ -1225,7 +1231,57 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
   ex_.steps.push_back(st);
 }
 
+// Few contiguous complex sequences of a length with a split instance: one
+// launch of P workgroups per sequence instead of one workgroup per sequence
+// (or a multi-pass chain).  Any explicit tuning override keeps the planner's
+// form, so the autotuner weighs both.
+bool Builder::try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  if (opt_.generic_kernels || opt_.forced_units != 0 || opt_.ept != 0 || !opt_.forced_passes.empty() ||
+      !opt_.forced_radices.empty())
+    return false;
+  if ((c.kind != CoreKind::Leaf && c.kind != CoreKind::MultiPass) || s.col_fast || s.ncol != 1 || s.gi.es != 1 ||
+      s.go.es != 1 || m.src_mode != kSrcComplex || m.dst_mode != kDstComplex)
+    return false;
+  const int id = leaf_split_kernel_id(c.n);
+  if (id < 0) return false;
+  const int ncu = cu_count();
+  if (s.nbatch * 2 > ncu) return false;
+  int p = 1, l = 1, ept = 0;
+  std::vector<int> rad;
+  leaf_split_shape(id, dbl(), p, l, ept, rad);
+  const int tpu = (l + ept - 1) / ept;
+  Step st;
+  st.kind = Step::LeafSplit;
+  st.dbl = dbl();
+  st.src = src;
+  st.dst = dst;
+  st.leaf_id = id;
+  st.split = p;
+  LeafSplitArgs &a = st.lsa;
+  a.nseq = s.nbatch;
+  a.idist = s.gi.bs;
+  a.odist = s.go.bs;
+  a.tw_n = roots(c.n, c.n, 1);
+  a.tw_l = roots(l, l, 1);
+  a.conj_in = m.conj_in;
+  a.conj_out = m.conj_out;
+  a.scale = m.scale;
+  const int64_t lds = (l - 1) + ((l - 1) >> 5) + 1;
+  st.ept = ept;
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(s.nbatch * p)), dim3(static_cast<unsigned>(tpu)),
+                         static_cast<size_t>(lds) * eb()};
+  st.bytes_moved = s.nbatch * c.n * eb() * 2;
+  std::ostringstream lab;
+  lab << "k_leaf_split<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << p << ">[";
+  for (size_t i = 0; i < rad.size(); ++i) lab << (i ? "x" : "") << rad[i];
+  lab << "] n=" << c.n << " transforms=" << s.nbatch;
+  st.label = lab.str();
+  ex_.steps.push_back(st);
+  return true;
+}
+
 void Builder::emit_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
+  if (try_leaf_split(c, s, src, dst, m)) return;
   switch (c.kind) {
     case CoreKind::Leaf: emit_leaf(c, s, src, dst, m); break;
     case CoreKind::MultiPass: emit_multipass(c, s, src, dst, m, m, {}); break;

@@ -47,6 +47,7 @@ struct Step {
     Bluestein,
     BluesteinLds,
     BluesteinCombine,
+    LeafSplit,
     R2CEven,
     C2REven,
     RealPost,
@@ -68,6 +69,7 @@ struct Step {
   MfmaArgs ma{};
   BluesteinLdsArgs ba{};
   BluesteinCombineArgs ca{};
+  LeafSplitArgs lsa{};
   int compute = 0;  // Mfma: kComputeBf16x3 / kComputeFp16x3
   LaunchShape shape{};
   int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)
@@ -193,6 +195,7 @@ class Builder {
   void emit_rader(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_bluestein_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m, int split);
+  bool try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
                    int64_t R);

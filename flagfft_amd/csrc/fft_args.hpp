@@ -130,6 +130,20 @@ struct LeafArgs {
   int32_t C;             // transforms per workgroup
 };
 
+// Single-pass transform of length N = P L spread over P workgroups per
+// sequence (contiguous complex rows): workgroup (q, h) computes
+// X[P k + h], k < L.
+struct LeafSplitArgs {
+  const void *in;
+  void *out;
+  int64_t nseq;
+  int64_t idist, odist;  // complex elements between consecutive transforms
+  const void *tw_n;      // w_N^j, j < N
+  const void *tw_l;      // w_L^j, j < L
+  int32_t conj_in, conj_out;
+  double scale;
+};
+
 // Compile-time specialised Stockham super-stage over contiguous complex
 // sequences (unit u -> (b = u / tcnt, i = u % tcnt)).
 struct StageArgs {
