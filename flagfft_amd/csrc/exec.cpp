@@ -1042,9 +1042,11 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     } else if (opt_.forced_units < 1 || opt_.forced_units > 5) {
       // measured on MI355X: a whole convolution per workgroup stays ahead
       // until fewer than 1/8 of the CUs have a sequence; the two-halves form
-      // (two convolutions in a row per workgroup) already below 1/2
+      // (two convolutions in a row per workgroup) splits in 2 below 1/2
+      // (in 4 below 1/8: the combine pass reads P blocks, so P = 2 costs
+      // less device time when the GPU is shared)
       const int ncu = cu_count();
-      if (c.lds_halves ? nseqs_all * 4 <= ncu : nseqs_all * 8 <= ncu)
+      if (nseqs_all * 8 <= ncu)
         split = 4;
       else if (c.lds_halves && nseqs_all * 2 <= ncu)
         split = 2;
