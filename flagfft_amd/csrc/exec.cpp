@@ -201,7 +201,7 @@ void Executable::resolve_kernels() {
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
-      case Step::LeafSplit: s.func = leaf_split_kernel(s.leaf_id, s.dbl); break;
+      case Step::LeafSplit: s.func = leaf_split_kernel(s.leaf_id, s.dbl, s.leaf_mode); break;
       case Step::Rader: s.func = rader_kernel(s.dbl, s.ept); break;
       case Step::Bluestein: s.func = bluestein_kernel(s.dbl, s.ept); break;
       case Step::R2CEven: s.func = r2c_even_kernel(s.dbl, s.ept); break;
@@ -1245,9 +1245,31 @@ bool Builder::try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, c
       s.go.es != 1 || m.src_mode != kSrcComplex || m.dst_mode != kDstComplex)
     return false;
   const int id = leaf_split_kernel_id(c.n);
-  if (id < 0) return false;
-  const int ncu = cu_count();
-  if (s.nbatch * 2 > ncu) return false;
+  if (id < 0 || s.nbatch * 2 > cu_count()) return false;
+  emit_leaf_split(id, kRowComplex, c.n, s.nbatch, s.gi.bs, s.go.bs, src, dst, m.conj_in, m.conj_out, m.scale);
+  return true;
+}
+
+// Real rows of even length n whose n/2-point core has a two-way split
+// instance, in small batches: the split leaf with the pack / unpack fused.
+bool Builder::try_leaf_split_real(const Plan1D &pl, const SeqSet &s, int src, int dst) {
+  if (opt_.generic_kernels || opt_.forced_units != 0 || opt_.ept != 0 || !opt_.forced_radices.empty())
+    return false;
+  const int64_t h = pl.n / 2;
+  if (s.ncol != 1 || s.col_fast || s.gi.es != 1 || s.go.es != 1 || h < 2 ||
+      (pl.real_fwd ? s.gi.bs % 2 != 0 : s.go.bs % 2 != 0))
+    return false;
+  const int id = leaf_split_kernel_id(h);
+  if (id < 0 || leaf_split_kernel(id, dbl(), pl.real_fwd ? kRowR2C : kRowC2R) == nullptr ||
+      s.nbatch * 2 > cu_count())
+    return false;
+  emit_leaf_split(id, pl.real_fwd ? kRowR2C : kRowC2R, h, s.nbatch, pl.real_fwd ? s.gi.bs / 2 : s.gi.bs,
+                  pl.real_fwd ? s.go.bs : s.go.bs / 2, src, dst, false, !pl.real_fwd, 1.0);
+  return true;
+}
+
+void Builder::emit_leaf_split(int id, int mode, int64_t n, int64_t nseq, int64_t idist, int64_t odist, int src,
+                              int dst, bool conj_in, bool conj_out, double scale) {
   int p = 1, l = 1, ept = 0;
   std::vector<int> rad;
   leaf_split_shape(id, dbl(), p, l, ept, rad);
@@ -1258,28 +1280,30 @@ bool Builder::try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, c
   st.src = src;
   st.dst = dst;
   st.leaf_id = id;
+  st.leaf_mode = mode;
   st.split = p;
   LeafSplitArgs &a = st.lsa;
-  a.nseq = s.nbatch;
-  a.idist = s.gi.bs;
-  a.odist = s.go.bs;
-  a.tw_n = roots(c.n, c.n, 1);
+  a.nseq = nseq;
+  a.idist = idist;
+  a.odist = odist;
+  a.tw_n = roots(n, n, 1);
   a.tw_l = roots(l, l, 1);
-  a.conj_in = m.conj_in;
-  a.conj_out = m.conj_out;
-  a.scale = m.scale;
+  a.aux = mode == kRowComplex ? nullptr : roots(n + 1, 2 * n, 1);
+  a.conj_in = conj_in;
+  a.conj_out = conj_out;
+  a.scale = scale;
   const int64_t lds = (l - 1) + ((l - 1) >> 5) + 1;
   st.ept = ept;
-  st.shape = LaunchShape{dim3(static_cast<unsigned>(s.nbatch * p)), dim3(static_cast<unsigned>(tpu)),
+  st.shape = LaunchShape{dim3(static_cast<unsigned>(nseq * p)), dim3(static_cast<unsigned>(tpu)),
                          static_cast<size_t>(lds) * eb()};
-  st.bytes_moved = s.nbatch * c.n * eb() * 2;
+  st.bytes_moved = mode == kRowComplex ? nseq * n * eb() * 2 : nseq * (n * eb() + (n + 1) * eb());
   std::ostringstream lab;
-  lab << "k_leaf_split<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << p << ">[";
+  lab << "k_leaf_split<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << p << "," << mode << ">[";
   for (size_t i = 0; i < rad.size(); ++i) lab << (i ? "x" : "") << rad[i];
-  lab << "] n=" << c.n << " transforms=" << s.nbatch;
+  lab << "] " << (mode == kRowComplex ? "C2C" : mode == kRowR2C ? "R2C" : "C2R") << " n=" << n
+      << " transforms=" << nseq;
   st.label = lab.str();
   ex_.steps.push_back(st);
-  return true;
 }
 
 void Builder::emit_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m) {
@@ -1319,6 +1343,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     return;
   }
   const int64_t h = n / 2;
+  if (pl.real_form == RealForm::EvenFused && try_leaf_split_real(pl, s, src, dst)) return;
   if (pl.real_form == RealForm::EvenFused) {
     const int lid = opt_.generic_kernels ? -1 : leaf_kernel_id(pl.core.radices);
     if (lid >= 0 && s.ncol == 1 && !s.col_fast && s.gi.es == 1 && s.go.es == 1 && h > 1 &&

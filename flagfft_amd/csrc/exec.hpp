@@ -196,6 +196,9 @@ class Builder {
   void emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void emit_bluestein_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m, int split);
   bool try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
+  bool try_leaf_split_real(const Plan1D &pl, const SeqSet &s, int src, int dst);
+  void emit_leaf_split(int id, int mode, int64_t n, int64_t nseq, int64_t idist, int64_t odist, int src, int dst,
+                       bool conj_in, bool conj_out, double scale);
   void emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst, const Modes &m);
   void fill_stages(int32_t *rad, FastDiv *fdq, int32_t *twstep, int32_t &nst, const std::vector<int> &radices,
                    int64_t R);

@@ -140,6 +140,7 @@ struct LeafSplitArgs {
   int64_t idist, odist;  // complex elements between consecutive transforms
   const void *tw_n;      // w_N^j, j < N
   const void *tw_l;      // w_L^j, j < L
+  const void *aux;       // real rows (2N reals): w_2N^k, k <= N
   int32_t conj_in, conj_out;
   double scale;
 };

@@ -72,12 +72,15 @@ std::vector<BluesteinLdsShape> bluestein_lds_candidates(bool dbl);
 int bluestein_split_kernel_id(int64_t m, bool dbl, int split);
 std::vector<int> bluestein_lds_kernel_radices(int id);
 const void *bluestein_combine_kernel(bool dbl, int split);
+// Row kinds of the single-pass kernels (the kernels' LeafMode values).
+constexpr int kRowComplex = 0, kRowR2C = 1, kRowC2R = 2;
+
 // Single-pass transforms spread over P workgroups per sequence (leaf_split.hip):
 // instance for length n (-1 when none), its split P, block length L = n / P,
 // elements per thread and block radices, and the kernel.
 int leaf_split_kernel_id(int64_t n);
 void leaf_split_shape(int id, bool dbl, int &p, int &l, int &ept, std::vector<int> &radices);
-const void *leaf_split_kernel(int id, bool dbl);
+const void *leaf_split_kernel(int id, bool dbl, int mode);  // mode: LeafMode (real modes: P = 2 instances)
 
 // Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).
 const void *mfma_kernel(int mode);

@@ -29,7 +29,6 @@ namespace dev {
 //                 z = conj FFT_N(conj Z') -> (x_2j, x_2j+1)
 // (python/flagfft_codegen/kernels.py:1098-1221 promotes real rows to full-length
 // complex transforms instead; the split-spectrum identity halves the work.)
-enum LeafMode : int { kLeafComplex = 0, kLeafR2C = 1, kLeafC2R = 2 };
 
 template <typename T, int E, int Mode, int... Rs>
 __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {

@@ -10,6 +10,10 @@
 namespace flagfft {
 namespace dev {
 
+// Row kinds of the single-pass kernels: complex rows, real rows in (R2C:
+// split-spectrum unpack fused), Hermitian rows in (C2R: pack fused).
+enum LeafMode : int { kLeafComplex = 0, kLeafR2C = 1, kLeafC2R = 2 };
+
 template <int... Rs>
 struct RadixSeq {
   static constexpr int kS = sizeof...(Rs);

@@ -57,3 +57,39 @@ def test_split_matches_unsplit(gpu):
     finally:
         ff.set_tuning()
     assert ora.rel_l2(split, whole, n)[0] < 2e-6
+
+
+REAL_CASES = [(6000, 128), (6000, 5), (6000, 1)]
+
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("n,batch", REAL_CASES, ids=[f"{n}x{b}" for n, b in REAL_CASES])
+def test_split_leaf_real(gpu, n, batch, dbl):
+    """Real rows of 6000 (3000-point core split in two, pack / unpack fused): R2C
+    against the oracle's half spectrum, C2R of a non-Hermitian input against the
+    oracle (Im of DC / Nyquist ignored), R2C -> C2R = n x."""
+    import torch
+
+    from .helpers import dev_tensor, host, real_input, stream_of, sync
+
+    tf, ti = (ff.Type.D2Z, ff.Type.Z2D) if dbl else (ff.Type.R2C, ff.Type.C2R)
+    assert _uses_split(n, tf, batch) and _uses_split(n, ti, batch)
+    h = n // 2 + 1
+    cdt = np.complex128 if dbl else np.complex64
+    x = real_input(ora.accuracy_seed(int(tf), n, batch, 11), (batch, n), np.float64 if dbl else np.float32)
+    X = complex_input(ora.accuracy_seed(int(ti), n, batch, 12), (batch, h), cdt)
+    xd = dev_tensor(x, gpu)
+    Xd = torch.zeros((batch, h), dtype=torch.complex128 if dbl else torch.complex64, device=gpu)
+    Xin = dev_tensor(X, gpu)
+    yd = xd.clone().zero_()
+    zd = xd.clone().zero_()
+    with ff.Plan.one_d(n, tf, batch) as pf, ff.Plan.one_d(n, ti, batch) as pi:
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        (pf.d2z if dbl else pf.r2c)(xd.data_ptr(), Xd.data_ptr())
+        (pi.z2d if dbl else pi.c2r)(Xd.data_ptr(), zd.data_ptr())
+        (pi.z2d if dbl else pi.c2r)(Xin.data_ptr(), yd.data_ptr())
+        sync(gpu)
+    check_accuracy(host(Xd), ora.rfft(x.astype(np.float64)), tf, n, row_len=h, what=f"R2C batch={batch}")
+    check_accuracy(host(yd), ora.irfft(X.astype(np.complex128), n), ti, n, row_len=n, what=f"C2R batch={batch}")
+    check_accuracy(host(zd), x.astype(np.float64) * n, ti, n, row_len=n, factor=2.0, what="roundtrip")
