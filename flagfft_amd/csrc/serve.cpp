@@ -38,6 +38,7 @@ constexpr size_t kAlign = 256;
 constexpr int kMaxJobs = 2048;                     // queue jobs per persistent launch
 constexpr int64_t kQueueMaxBytes = int64_t{32} << 20;  // larger single-pass requests launch on their own
 constexpr int64_t kNominalBatch = 512;             // batch the cached plans are shaped for
+constexpr double kLargeRequestBytes = 64.0 * (1 << 20);  // direct requests above this stay in arrival order
 constexpr int64_t kQueueNominal = int64_t{1} << 20;  // queue templates: full tiles (C * tpu ~ 256 threads)
 
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -121,11 +122,14 @@ struct Request {
 struct Group {  // requests completed by one event
   hipEvent_t ev = nullptr;
   std::vector<Request *> reqs;
+  bool queue = false;  // a persistent-kernel launch (the slot's job staging is in use until it completes)
+  double bytes = 0.0;  // algorithmic bytes, counted in the slot's load
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
-  bool busy = false;
+  bool busy = false;  // job staging in use by a queue launch
+  double load = 0.0;  // algorithmic bytes launched and not yet completed
   std::vector<Group> groups;
   QueueJob *host_jobs = nullptr;  // pinned staging: counter block then jobs
   void *dev_jobs = nullptr;
@@ -149,7 +153,10 @@ bool queue_kind_of(const Step &s, int &kind) {
 class Server {
  public:
   Server(size_t budget, int max_inflight, int mode)
-      : arena_(budget), mode_(mode), nslots_(mode == FLAGFFTX_SERVE_PLAIN ? 1 : std::max(1, max_inflight)) {
+      : arena_(budget),
+        mode_(mode),
+        nslots_(mode == FLAGFFTX_SERVE_PLAIN ? 1 : std::max(1, max_inflight) + 1),
+        ndirect_(mode == FLAGFFTX_SERVE_PLAIN ? 1 : 2) {
     hip_check(hipGetDevice(&device_), "hipGetDevice");
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
@@ -391,10 +398,12 @@ class Server {
           --inflight_;
         }
         free_events_.push_back(s.groups[k].ev);
+        s.load -= s.groups[k].bytes;
+        if (s.groups[k].queue) s.busy = false;
         any = true;
       }
       if (k) s.groups.erase(s.groups.begin(), s.groups.begin() + static_cast<std::ptrdiff_t>(k));
-      if (s.groups.empty()) s.busy = false;
+      if (s.groups.empty()) s.load = 0.0;
     }
     return any;
   }
@@ -443,24 +452,38 @@ class Server {
         if (stop_) break;
       }
       if (stop_ && pending_.empty() && admitted_.empty() && inflight_ == 0) break;
-      // Requests that run their own plan's kernels go, in arrival order, to
-      // lane 0 as soon as they are admitted; single-pass requests wait for a
-      // free batch slot (its pinned job staging must be idle) and leave
-      // together in one persistent-kernel launch.  With one slot, lane 0 is
-      // also the batch slot.
-      std::vector<Request *> direct, queued;
+      // Requests that run their own plan's kernels go to a direct lane
+      // (slots [0, ndirect_)) as soon as they are admitted: large ones in
+      // arrival order on lane 0, smaller ones on the less loaded lane, so
+      // they are not held behind a large one; single-pass requests wait for a batch slot
+      // (slots [ndirect_, nslots_), never behind a direct request) whose
+      // pinned job staging is idle and leave together in one
+      // persistent-kernel launch.  PLAIN mode has one slot: every request
+      // runs its own plan on it, in arrival order.
+      std::vector<std::vector<Request *>> direct(slots_.size());
+      std::vector<Request *> queued;
       Slot *batch_slot = nullptr;
+      bool any_direct = false;
       if (!failed_ && !admitted_.empty()) {
-        for (size_t si = (slots_.size() > 1 ? 1 : 0); si < slots_.size(); ++si)
-          if (!slots_[si].busy) {
+        const size_t first_batch = slots_.size() > static_cast<size_t>(ndirect_) ? static_cast<size_t>(ndirect_) : 0;
+        for (size_t si = first_batch; si < slots_.size(); ++si)
+          if (!slots_[si].busy && (batch_slot == nullptr || slots_[si].load < batch_slot->load))
             batch_slot = &slots_[si];
-            break;
-          }
         for (auto it = admitted_.begin(); it != admitted_.end();) {
           Request *r = *it;
           const bool q = r->queued && mode_ == FLAGFFTX_SERVE_BATCHED;
           if (!q) {
-            direct.push_back(r);
+            // large requests keep arrival order on lane 0 (running two at once
+            // only stretches both); the others take the less loaded lane, so
+            // they pass the large ones
+            size_t best = 0;
+            const size_t nd = std::min(slots_.size(), static_cast<size_t>(ndirect_));
+            if (request_bytes(r) <= kLargeRequestBytes)
+              for (size_t si = 1; si < nd; ++si)
+                if (slots_[si].load < slots_[best].load) best = si;
+            slots_[best].load += request_bytes(r);
+            direct[best].push_back(r);
+            any_direct = true;
             it = admitted_.erase(it);
           } else if (batch_slot != nullptr && queued.size() < static_cast<size_t>(kMaxJobs)) {
             queued.push_back(r);
@@ -470,20 +493,26 @@ class Server {
           }
         }
       }
-      if (!direct.empty() || !queued.empty()) {
-        for (Request *r : direct) r->state = Request::InFlight;
-        for (Request *r : queued) r->state = Request::InFlight;
-        inflight_ += static_cast<int64_t>(direct.size() + queued.size());
-        Slot &lane = slots_[0];
-        if (!direct.empty()) lane.busy = true;
+      if (any_direct || !queued.empty()) {
+        for (auto &d : direct)
+          for (Request *r : d) r->state = Request::InFlight;
+        for (Request *r : queued) {
+          r->state = Request::InFlight;
+          batch_slot->load += request_bytes(r);
+        }
+        for (auto &d : direct) inflight_ += static_cast<int64_t>(d.size());
+        inflight_ += static_cast<int64_t>(queued.size());
         if (!queued.empty()) batch_slot->busy = true;
         lk.unlock();
-        std::vector<Group> lane_groups, batch_groups;
+        std::vector<std::vector<Group>> groups(slots_.size());
+        std::vector<Group> batch_groups;
         hipError_t e = hipSuccess;
-        if (!direct.empty()) e = launch(lane, direct, lane_groups);
+        for (size_t si = 0; si < slots_.size() && e == hipSuccess; ++si)
+          if (!direct[si].empty()) e = launch(slots_[si], direct[si], groups[si]);
         if (e == hipSuccess && !queued.empty()) e = launch(*batch_slot, queued, batch_groups);
         lk.lock();
-        for (Group &g : lane_groups) lane.groups.push_back(std::move(g));
+        for (size_t si = 0; si < slots_.size(); ++si)
+          for (Group &g : groups[si]) slots_[si].groups.push_back(std::move(g));
         if (!queued.empty())
           for (Group &g : batch_groups) batch_slot->groups.push_back(std::move(g));
         if (e != hipSuccess) {
@@ -566,7 +595,11 @@ class Server {
       if (e != hipSuccess) return e;
       Group g;
       g.ev = take_event_unlocked();
-      for (const Item &it : items) g.reqs.push_back(it.r);
+      g.queue = true;
+      for (const Item &it : items) {
+        g.reqs.push_back(it.r);
+        g.bytes += request_bytes(it.r);
+      }
       e = hipEventRecord(g.ev, slot.stream);
       groups.push_back(std::move(g));
       if (e != hipSuccess) return e;
@@ -582,6 +615,7 @@ class Server {
       Group g;
       g.ev = take_event_unlocked();
       g.reqs.push_back(r);
+      g.bytes = request_bytes(r);
       e = hipEventRecord(g.ev, slot.stream);
       groups.push_back(std::move(g));
       if (e != hipSuccess) return e;
@@ -589,6 +623,10 @@ class Server {
     std::lock_guard<std::mutex> lk(mu_);
     ++st_batches_;
     return hipSuccess;
+  }
+
+  static double request_bytes(const Request *r) {
+    return static_cast<double>(r->batch) * static_cast<double>(r->plan->in_bytes + r->plan->out_bytes);
   }
 
   hipEvent_t take_event_unlocked() {
@@ -608,6 +646,7 @@ class Server {
   Arena arena_;
   int mode_;
   int nslots_;
+  int ndirect_;  // slots [0, ndirect_) run requests with their own plans; the rest take queue batches
   int device_ = 0;
   int ncu_ = 1;
   std::vector<Slot> slots_;

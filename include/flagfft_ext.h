@@ -109,11 +109,13 @@ flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType typ
  * input must stay valid until the request completes) and returns at once.  A
  * request is admitted when its output and workspace fit in the arena (FIFO),
  * so requests that arrive while others run start as memory frees up.
- *   BATCHED: admitted requests are formed into batches, one per free stream
- *     slot (up to max_inflight batches overlap on the device).  Single-pass
- *     requests of a batch run together in one persistent mixed-length kernel
- *     launch that drains a tile queue; longer or larger requests run their own
- *     plan's kernels in the same batch.
+ *   BATCHED: admitted single-pass requests are formed into batches, one per
+ *     free batch stream (up to max_inflight batches overlap on the device),
+ *     and run together in one persistent mixed-length kernel launch that
+ *     drains a tile queue; longer or larger requests run their own plan's
+ *     kernels on two further streams: those above 64 MiB in arrival order on
+ *     the first, the others on the less loaded one, so they pass the large
+ *     ones and overlap with the batches.
  *   PLAIN: each admitted request is launched on its own, in arrival order, on
  *     one stream, with its cached plan (what per-request Exec calls do).
  * The output stays in the arena until Release.  Latency is measured from
