@@ -55,8 +55,23 @@ class Case:
         cdt = torch.complex128 if dbl else torch.complex64
         rdt = torch.float64 if dbl else torch.float32
         esz = 16 if dbl else 8
-        h = n // 2 + 1
         T = ff.Type
+        if isinstance(n, tuple):  # 2D (n0 x n1, row-major), one transform per exec
+            n0, n1 = n
+            h = n1 // 2 + 1
+            self.batch = 1
+            if kind == "c2c":
+                self.plans = [ff.Plan.two_d(n0, n1, T.Z2Z if dbl else T.C2C)]
+                self.alg_bytes = n0 * n1 * esz * 2
+                self.shapes = [((n0, n1), cdt), ((n0, n1), cdt)]
+            else:
+                self.plans = [ff.Plan.two_d(n0, n1, T.D2Z if dbl else T.R2C),
+                              ff.Plan.two_d(n0, n1, T.Z2D if dbl else T.C2R)]
+                self.alg_bytes = 2 * (n0 * n1 * esz // 2 + n0 * h * esz)
+                self.shapes = [((n0, n1), rdt), ((n0, h), cdt), ((n0, n1), rdt)]
+            self.sets = []
+            return
+        h = n // 2 + 1
         if kind == "c2c":
             self.plans = [ff.Plan.one_d_compute(n, T.Z2Z if dbl else T.C2C, batch, compute)]
             self.alg_bytes = batch * n * esz * 2
@@ -138,6 +153,8 @@ SUITE = [  # the other configurations, reported with --suite (not the headline l
     [("C2C f32 N=1024 x65536 FP16 hi+lo matrix cores", "c2c", 1024, 65536, False, 2)],
     [("C2C f32 N=1024 x65536 BF16 hi+lo matrix cores", "c2c", 1024, 65536, False, 1)],
     [("R2C->C2R f32 N=2^20 x16", "real", 1 << 20, 16, False)],
+    [("2D R2C->C2R f32 4096x4096", "real", (4096, 4096), 1, False)],
+    [("2D Z2Z f64 2048x2048", "c2c", (2048, 2048), 1, True)],
 ]
 
 
@@ -456,6 +473,10 @@ def accuracy_sample(case, torch, rows: int = 64):
     b = case.sets[0]
     case.run(0)
     torch.cuda.synchronize()
+    if isinstance(case.n, tuple):  # 2D: the whole transform
+        x = b[0].cpu().numpy().astype(np.complex128)
+        y = b[1].cpu().numpy()
+        return float(f"{ora.rel_l2(y.reshape(1, -1), ora.fft2(x[None]).reshape(1, -1))[0]:.3e}")
     x = b[0][:rows].cpu().numpy().astype(np.complex128)
     y = b[1][:rows].cpu().numpy()
     return float(f"{ora.rel_l2(y, ora.fft(x))[0]:.3e}")

@@ -617,6 +617,18 @@ void Builder::emit_leaf(const Core &c, const SeqSet &s, int src, int dst, const 
     ex_.steps.push_back(st);
     return;
   }
+  if (s.col_fast && m.src_mode == kSrcComplex && m.dst_mode == kDstComplex && c.n > 1) {
+    // columns of contiguous rows: a specialised super-stage with the column
+    // index as the fastest unit digit (no inter-pass twiddle)
+    TwoLevel tl;
+    tl.lo = tl.hi = roots(c.n, c.n, 1);
+    PassArgs a = chain_pass(c.radices, static_cast<int>(c.n), 1, c.n, s, s.gi, s.go, tl);
+    a.conj_in = m.conj_in;
+    a.conj_out = m.conj_out;
+    a.scale = m.scale;
+    a.nlog = m.nlog ? m.nlog : c.n;
+    if (try_stage(a, c.radices, 0, src, dst, s, c.n, 0, 1)) return;
+  }
   Step st;
   st.kind = Step::Pass;
   st.dbl = dbl();
@@ -720,11 +732,16 @@ Builder::TwoLevel Builder::two_level(int64_t n) {
 bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst,
                         const SeqSet &s, int64_t n, int pass, int npasses) {
   const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
-  if (sid < 0 || s.col_fast || s.ncol != 1 || a.gi.es != 1 || a.go.es != 1 || (a.src_mode & 3) != kSrcComplex ||
-      (a.dst_mode & 3) != kDstComplex)
-    return false;
+  if (sid < 0 || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
+  // contiguous sequences, or the columns of contiguous rows of ncol elements
+  const bool rows = !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1;
+  const bool cols = s.col_fast && s.ncol > 1 && a.gi.es == s.ncol && a.gi.cs == 1 && a.go.es == s.ncol &&
+                    a.go.cs == 1 && mode == 0 && a.src_mode == kSrcComplex && a.dst_mode == kDstComplex;
+  if (!rows && !cols) return false;
+  const int64_t ncol = cols ? s.ncol : 1;
   const int R = a.R;
-  const int64_t tcnt = a.tcnt, p = a.p;
+  const int64_t tcnt = a.tcnt * ncol, p = a.p * ncol;
+  if (tcnt >= (int64_t{1} << 31)) return false;
   int cu = 0, ept = 0;
   stage_kernel_shape(sid, dbl(), cu, ept);
   const int tpu = (R + ept - 1) / ept;
@@ -745,6 +762,8 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   g.tw_bits = a.tw_bits;
   g.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
   g.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+  g.ncol = static_cast<int32_t>(ncol);
+  g.fd_ncol = make_fastdiv(static_cast<uint32_t>(ncol));
   g.tw_small = a.tw_small;
   g.tw_lo = a.tw_lo;
   g.tw_hi = a.tw_hi;
@@ -752,7 +771,7 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   g.conj_out = a.conj_out;
   g.scale = a.scale;
   g.units = a.units;
-  g.block_store = (mode == 1 || (p == 1 && a.dst_mode == kDstComplex && a.go.bs == n)) ? 1 : 0;
+  g.block_store = (mode == 1 || (rows && p == 1 && a.dst_mode == kDstComplex && a.go.bs == n)) ? 1 : 0;
   if (mode == 1 && a.go.bs != n) throw std::runtime_error("convolution pass needs contiguous sequences");
   g.src_mode = a.src_mode;
   g.dst_mode = a.dst_mode;
@@ -765,12 +784,14 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   sst.ept = ept;
   sst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cu * tpu)),
                           static_cast<size_t>(cu) * LS * eb()};
-  sst.bytes_moved = s.nbatch * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+  sst.bytes_moved =
+      s.nbatch * ncol * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
   std::ostringstream lab;
   lab << "k_stage<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << cu << "," << mode << ">[";
   for (size_t ri = 0; ri < radices.size(); ++ri) lab << (ri ? "x" : "") << radices[ri];
   lab << "] " << (mode == 1 ? "convolution" : "pass") << " " << pass << "/" << npasses << " R=" << R << " p=" << p
-      << " tcnt=" << tcnt << (g.block_store ? " block-store" : "");
+      << " tcnt=" << tcnt << (g.block_store ? " block-store" : "") << (cols ? " columns=" : "");
+  if (cols) lab << ncol;
   sst.label = lab.str();
   ex_.steps.push_back(sst);
   return true;

@@ -145,14 +145,20 @@ struct LeafSplitArgs {
   double scale;
 };
 
-// Compile-time specialised Stockham super-stage over contiguous complex
-// sequences (unit u -> (b = u / tcnt, i = u % tcnt)).
+// Compile-time specialised Stockham super-stage over complex sequences
+// (unit u -> (b = u / tcnt, i = u % tcnt)).  Sequences are contiguous
+// (ncol = 1), or the ncol columns of row-major rows of ncol elements: the
+// column index is then the fastest digit of i, tcnt and p count
+// column-interleaved units (ncol times the per-column figures), and element
+// e of column c sits at c + e ncol.
 struct StageArgs {
   const void *in;
   void *out;
   int64_t bs_in, bs_out;  // complex elements between sequences
   int32_t tcnt, p, twp_mult, tw_bits;
   FastDiv fd_tcnt, fd_p;
+  int32_t ncol;
+  FastDiv fd_ncol;
   const void *tw_small;   // w_R^j
   const void *tw_lo;      // w_nseq^l, l < 2^tw_bits
   const void *tw_hi;      // w_nseq^(h 2^tw_bits)

@@ -155,6 +155,14 @@ bool split_passes(int64_t n, int k, int64_t cap, std::vector<int64_t> &best) {
   return true;
 }
 
+// Row-segment bytes of the widest specialised column super-stage for a radix
+// sequence (adjacent columns per workgroup x element bytes); 0 if none.
+int column_segment_bytes(const std::vector<int> &rad, Prec p) {
+  for (int units : {64, 32, 16, 8, 4, 2, 1})
+    if (stage_kernel_id(rad, p == Prec::F64, units) >= 0) return units * elem_bytes(p);
+  return 0;
+}
+
 bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c) {
   std::vector<int64_t> passes;
   const int64_t cap = pass_max(o, p);
@@ -200,7 +208,10 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   c.n = n;
   std::vector<int> rad;
   const int64_t lmax = leaf_max(o, p);
-  bool leaf = n <= lmax && radix_factor(n, rad);
+  // an explicit super-radix split of exactly n overrides the single pass
+  int64_t forced_prod = o.forced_passes.size() >= 2 ? 1 : 0;
+  for (int64_t r : o.forced_passes) forced_prod *= r;
+  bool leaf = n <= lmax && radix_factor(n, rad) && forced_prod != n;
   if (!leaf && o.forced_passes.empty() && n <= 32768 && radix_factor(n, rad) && leaf_kernel_id(rad) >= 0 &&
       static_cast<int64_t>(lds_pad(static_cast<int>(n) - 1) + 2) * elem_bytes(p) <= o.lds_cap_bytes) {
     // a specialised instance holding the whole sequence in LDS: one HBM round
@@ -389,6 +400,31 @@ PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
   } else {
     t.rows = plan_1d(d.n1, d.real_fwd, d.real_inv, d.prec, o);
     t.cols = plan_1d(d.n0, false, false, d.prec, o);
+    // A column super-stage reads row segments of (adjacent columns per
+    // workgroup) x (element bytes).  When the single pass's widest
+    // specialised tile gives segments under 64 B (a long column filling the
+    // LDS with few columns), two passes of wide tiles move the data twice at
+    // full segment width and win (4096-point FP32 columns: 66 us against
+    // 108 us per 2049 x 4096 half spectrum).
+    const bool tuned = !o.forced_passes.empty() || !o.forced_radices.empty() || o.forced_units != 0 ||
+                       o.generic_kernels;
+    if (!tuned && t.cols.core.kind == CoreKind::Leaf) {
+      const int seg = column_segment_bytes(t.cols.core.radices, d.prec);
+      Core mc;
+      if (seg > 0 && seg < 64 && split_passes(d.n0, 2, pass_max(o, d.prec), mc.passes)) {
+        bool wide = true;
+        for (int64_t r : mc.passes) {
+          std::vector<int> rad;
+          wide = wide && radix_factor(r, rad) && column_segment_bytes(rad, d.prec) >= 64;
+          mc.pass_radices.push_back(rad);
+        }
+        if (wide) {
+          mc.kind = CoreKind::MultiPass;
+          mc.n = d.n0;
+          t.cols.core = mc;
+        }
+      }
+    }
   }
   return t;
 }

@@ -18,8 +18,10 @@ namespace dev {
 //   kStagePass         one pass of a chain: unit u = b * tcnt + i of sequence b
 //                      reads x[i + e tcnt] (e < R), multiplies by
 //                      w_{pR}^{e (i mod p)}, transforms and writes
-//                      y[(i - k) R + k + e p]; Bluestein chains use the chirp
-//                      source (first pass) and the pointwise / chirp sinks.
+//                      y[(i - k) R + k + e p], k = i mod p; Bluestein chains
+//                      use the chirp source (first pass) and the pointwise /
+//                      chirp sinks.  Column sequences (ncol > 1) run the same
+//                      index map over column-interleaved units.
 //   kStageConvolution  middle of a two-pass Bluestein convolution M = R0 * R:
 //                      the forward chain's second pass (p = tcnt = R0), the
 //                      pointwise conj(X * bhat), and the inverse chain's first
@@ -50,8 +52,11 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage(const StageArgs a) {
   const uint32_t uu = active ? static_cast<uint32_t>(u) : 0u;
   const uint32_t bq = fdiv(uu, a.fd_tcnt);
   const uint32_t i = uu - bq * static_cast<uint32_t>(a.tcnt);
-  const bool first = a.p == 1;
-  const uint32_t k = first ? 0u : i - fdiv(i, a.fd_p) * static_cast<uint32_t>(a.p);
+  // first pass: p is one per column; k: position inside the prior product
+  // (column-interleaved), kt = k / ncol its per-column twiddle exponent
+  const bool first = a.p == a.ncol;
+  const uint32_t k = i - fdiv(i, a.fd_p) * static_cast<uint32_t>(a.p);
+  const uint32_t kt = first ? 0u : fdiv(k, a.fd_ncol);
   const int64_t tcnt = a.tcnt;
   const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in;
   const GlbC<T> *tws = gc<T>(a.tw_small);
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_stage(const StageArgs a) {
   const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
   const GlbC<T> *chirp = gc<T>(a.chirp);
   const uint32_t tw_mask = (1u << a.tw_bits) - 1u;
-  const uint32_t kmul = k * static_cast<uint32_t>(a.twp_mult);
+  const uint32_t kmul = kt * static_cast<uint32_t>(a.twp_mult);
   const T scale = static_cast<T>(a.scale);
   const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
   LdsC<T> *buf = lds + sl * LS;

@@ -50,6 +50,41 @@ def test_c2c_2d(gpu, shape, dbl):
         check_accuracy(host(zd), x.astype(np.complex128) * n, t, n, row_len=n, factor=4.0, what="2D roundtrip")
 
 
+
+# Column passes split into super-stages (the explicit split applies to the
+# column length only): specialised column super-stages with inter-pass
+# twiddles, mixed with run-time-radix passes where a super-radix has no
+# specialised instance (16).
+CHAINS = [((4096, 48), False, [64, 64]), ((8192, 20), False, [128, 64]), ((2048, 40), True, [16, 128]),
+          ((1024, 36), True, [64, 16]), ((4096, 24), True, [16, 16, 16])]
+
+
+@pytest.mark.parametrize("shape,dbl,passes", CHAINS, ids=[f"{s[0]}x{s[1]}-{'z2z' if d else 'c2c'}-{'x'.join(map(str, p))}"
+                                                          for s, d, p in CHAINS])
+def test_column_chains(gpu, shape, dbl, passes):
+    n0, n1 = shape
+    t = ff.Type.Z2Z if dbl else ff.Type.C2C
+    batch = 2
+    x = complex_input(n0 + 31 * n1, (batch, n0, n1), np.complex128 if dbl else np.complex64)
+    xd = dev_tensor(x, gpu)
+    yd = xd.clone().zero_()
+    zd = xd.clone().zero_()
+    ff.set_tuning(passes=passes)
+    try:
+        p = _plan(n0, n1, t, batch)
+    finally:
+        ff.set_tuning()
+    with p:
+        assert "MultiPassStockham(n=%d" % n0 in p.description()
+        p.set_stream(stream_of(gpu))
+        ex = p.z2z if dbl else p.c2c
+        ex(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
+        ex(yd.data_ptr(), zd.data_ptr(), ff.INVERSE)
+        sync(gpu)
+    n = n0 * n1
+    check_accuracy(host(yd), ora.fft2(x.astype(np.complex128)), t, n, row_len=n, factor=2.0, what="2D fwd")
+    check_accuracy(host(zd), x.astype(np.complex128) * n, t, n, row_len=n, factor=4.0, what="2D roundtrip")
+
 @pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("shape", SHAPES + [(33, 15)], ids=lambda s: f"{s[0]}x{s[1]}")
 def test_real_2d(gpu, shape, dbl):
