@@ -44,6 +44,7 @@ struct PlanImpl {
   int device = 0;
   PlanTree tree;
   std::unique_ptr<Executable> fwd, inv;
+  std::unique_ptr<Executable> fwd_ip, inv_ip;  // in-place calls of executables with an in-place hazard
   hipStream_t stream = nullptr;
   bool graph_mode = false;  // Exec replays one hipGraph per direction
   std::string description;
@@ -217,11 +218,17 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
       impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist);
       impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist);
     }
+    PlanOptions ip = opts;
+    ip.in_place = true;
+    if (impl->fwd && impl->fwd->inplace_hazard) impl->fwd_ip = build_executable(impl->tree, false, ip, d.idist, d.odist);
+    if (impl->inv && impl->inv->inplace_hazard) impl->inv_ip = build_executable(impl->tree, true, ip, d.idist, d.odist);
     std::ostringstream o;
     o << impl->tree.describe();
     if (!tuned.empty()) o << "tuned-plan: " << tuned << "\n";
     o << "\n-- Forward execution --\n" << (impl->fwd ? impl->fwd->describe() : std::string("(not compiled)\n"));
     if (impl->inv) o << "\n-- Inverse execution --\n" << impl->inv->describe();
+    if (impl->fwd_ip) o << "\n-- Forward execution, in place --\n" << impl->fwd_ip->describe();
+    if (impl->inv_ip) o << "\n-- Inverse execution, in place --\n" << impl->inv_ip->describe();
     impl->description = o.str();
     auto handle = std::make_unique<flagfftPlan_t>();
     handle->impl = impl.release();
@@ -255,6 +262,8 @@ flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagf
   if (p->desc.real_fwd) ex = p->fwd.get();
   else if (p->desc.real_inv) ex = p->inv.get();
   else ex = direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
+  if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
+  if (in == out && ex->inplace_hazard) ex = ex == p->fwd.get() ? p->fwd_ip.get() : p->inv_ip.get();
   if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
   int cur = p->device;
   if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);

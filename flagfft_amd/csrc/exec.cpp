@@ -1265,6 +1265,7 @@ bool Builder::try_leaf_split(const Core &c, const SeqSet &s, int src, int dst, c
   if ((c.kind != CoreKind::Leaf && c.kind != CoreKind::MultiPass) || s.col_fast || s.ncol != 1 || s.gi.es != 1 ||
       s.go.es != 1 || m.src_mode != kSrcComplex || m.dst_mode != kDstComplex)
     return false;
+  if (opt_.in_place && src == kBufIn && dst == kBufOut) return false;
   const int id = leaf_split_kernel_id(c.n);
   if (id < 0 || s.nbatch * 2 > cu_count()) return false;
   emit_leaf_split(id, kRowComplex, c.n, s.nbatch, s.gi.bs, s.go.bs, src, dst, m.conj_in, m.conj_out, m.scale);
@@ -1278,7 +1279,7 @@ bool Builder::try_leaf_split_real(const Plan1D &pl, const SeqSet &s, int src, in
     return false;
   const int64_t h = pl.n / 2;
   if (s.ncol != 1 || s.col_fast || s.gi.es != 1 || s.go.es != 1 || h < 2 ||
-      (pl.real_fwd ? s.gi.bs % 2 != 0 : s.go.bs % 2 != 0))
+      (pl.real_fwd ? s.gi.bs % 2 != 0 : s.go.bs % 2 != 0) || (opt_.in_place && src == kBufIn && dst == kBufOut))
     return false;
   const int id = leaf_split_kernel_id(h);
   if (id < 0 || leaf_split_kernel(id, dbl(), pl.real_fwd ? kRowR2C : kRowC2R) == nullptr ||
@@ -1294,6 +1295,8 @@ void Builder::emit_leaf_split(int id, int mode, int64_t n, int64_t nseq, int64_t
   int p = 1, l = 1, ept = 0;
   std::vector<int> rad;
   leaf_split_shape(id, dbl(), p, l, ept, rad);
+  // each of the P workgroups of a sequence reads all of it
+  if (src == kBufIn && dst == kBufOut) ex_.inplace_hazard = true;
   const int tpu = (l + ept - 1) / ept;
   Step st;
   st.kind = Step::LeafSplit;
@@ -1535,18 +1538,22 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
       c.col_fast = true;
       b.emit_plan1d(t.cols, c, kBufOut, kBufOut, inverse);
     } else if (t.desc.real_fwd) {
+      // rows into a workspace (an in-place call would otherwise rewrite the
+      // buffer at the complex row pitch while other rows are still read),
+      // columns from it into the output: the same two sweeps
+      const int W1 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
       SeqSet r;
       r.nbatch = batch * n0;
       r.gi = Geometry{1, 0, n1};
       r.go = Geometry{1, 0, h1};
-      b.emit_plan1d(t.rows, r, kBufIn, kBufOut, false);
+      b.emit_plan1d(t.rows, r, kBufIn, W1, false);
       SeqSet c;
       c.nbatch = batch;
       c.ncol = h1;
       c.gi = Geometry{h1, 1, n0 * h1};
       c.go = c.gi;
       c.col_fast = true;
-      b.emit_plan1d(t.cols, c, kBufOut, kBufOut, false);
+      b.emit_plan1d(t.cols, c, W1, kBufOut, false);
     } else {
       const int W2 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
       SeqSet c;

@@ -108,6 +108,10 @@ class Executable {
   // (ws[i] of ws_bytes[i] / built_batch * batch bytes).
   hipError_t run_batch(const void *in, void *out, void *const *ws, int64_t batch, hipStream_t st) const;
   int64_t built_batch = 1;
+  // A launch reads input that another workgroup of the same launch
+  // overwrites when the input and output are one buffer (split leaves): an
+  // in-place call needs the executable built with PlanOptions::in_place.
+  bool inplace_hazard = false;
   std::string describe() const;
   void allocate_workspaces();
 

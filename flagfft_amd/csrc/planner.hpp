@@ -36,6 +36,7 @@ struct PlanOptions {
   int forced_units = 0;                // explicit units per workgroup (tuned plans)
   int max_threads = 512;               // workgroup size limit of the launches
   bool generic_kernels = false;        // only the run-time-radix kernels (persistent-queue jobs)
+  bool in_place = false;               // input and output are one buffer: no sequence split over workgroups
 };
 
 enum class CoreKind { Leaf, MultiPass, Rader, Bluestein, MatrixCore };

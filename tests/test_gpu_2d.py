@@ -85,6 +85,36 @@ def test_column_chains(gpu, shape, dbl, passes):
     check_accuracy(host(yd), ora.fft2(x.astype(np.complex128)), t, n, row_len=n, factor=2.0, what="2D fwd")
     check_accuracy(host(zd), x.astype(np.complex128) * n, t, n, row_len=n, factor=4.0, what="2D roundtrip")
 
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("shape", [(64, 64), (67, 32), (256, 130), (4096, 4096)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_real_2d_in_place(gpu, shape, dbl):
+    """R2C then C2R with the input and output in one buffer of max(in, out) bytes."""
+    import torch
+
+    n0, n1 = shape
+    if dbl and n0 * n1 > 1 << 20:
+        pytest.skip("the 4096^2 case runs in FP32")
+    h = n1 // 2 + 1
+    tf, ti = (ff.Type.D2Z, ff.Type.Z2D) if dbl else (ff.Type.R2C, ff.Type.C2R)
+    rdt = np.float64 if dbl else np.float32
+    x = real_input(n0 * 3 + n1, (1, n0, n1), rdt, kind="normal")
+    buf = torch.zeros(2 * n0 * h, dtype=torch.float64 if dbl else torch.float32, device=gpu)
+    buf[: n0 * n1] = torch.from_numpy(x.reshape(-1)).to(gpu)
+    with ff.Plan.two_d(n0, n1, tf) as pf, ff.Plan.two_d(n0, n1, ti) as pi:
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        (pf.d2z if dbl else pf.r2c)(buf.data_ptr(), buf.data_ptr())
+        sync(gpu)
+        X = host(buf).view(np.complex128 if dbl else np.complex64).reshape(1, n0, h)
+        (pi.z2d if dbl else pi.c2r)(buf.data_ptr(), buf.data_ptr())
+        sync(gpu)
+        z = host(buf)[: n0 * n1].reshape(1, n0, n1)
+    n = n0 * n1
+    check_accuracy(X, ora.rfft2(x.astype(np.float64)), tf, n, row_len=n0 * h, what="in-place R2C")
+    check_accuracy(z, x.astype(np.float64) * n, ti, n, row_len=n, factor=2.0, what="in-place roundtrip")
+
+
 @pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
 @pytest.mark.parametrize("shape", SHAPES + [(33, 15)], ids=lambda s: f"{s[0]}x{s[1]}")
 def test_real_2d(gpu, shape, dbl):

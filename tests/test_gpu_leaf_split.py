@@ -39,6 +39,11 @@ def _uses_split(n, t, batch):
 def test_split_leaf(gpu, n, batch, dtype):
     t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
     assert _uses_split(n, t, batch)
+    with ff.Plan.one_d(n, t, batch) as p:
+        # in-place calls run an executable without the split (each of the
+        # split's workgroups reads the whole sequence)
+        ip = p.description().split("-- Forward execution, in place --")
+        assert len(ip) == 2 and "k_leaf_split<" not in ip[1].split("-- Inverse")[0]
     x = complex_input(ora.accuracy_seed(int(t), n, batch, 9), (batch, n), dtype, "normal")
     for direction, inplace in ((ff.FORWARD, False), (ff.INVERSE, False), (ff.FORWARD, True)):
         y = run_c2c(gpu, x, direction, n, batch, inplace=inplace)
