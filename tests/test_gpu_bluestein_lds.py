@@ -116,3 +116,54 @@ def test_real_sources(gpu, n, dbl):
         sync(gpu)
     check_accuracy(host(Xd), ora.rfft(x.astype(np.float64)), tf, n, row_len=n // 2 + 1, what="R2C")
     check_accuracy(host(zd), x.astype(np.float64) * n, ti, n, row_len=n, factor=2.0, what="roundtrip")
+
+
+SPLITS = [(4099, False), (9001, False), (12011, False), (20011, False),
+          (2053, True), (4099, True), (7919, True), (10007, True)]
+
+
+@pytest.mark.parametrize("n,dbl", SPLITS, ids=[f"{'z2z' if d else 'c2c'}-{n}" for n, d in SPLITS])
+def test_split_forms(gpu, n, dbl):
+    """Convolution spread over 2 / 4 workgroups per sequence (units 6 / 7) plus
+    the combine launch: both directions, several execs in a row, in place and
+    as graph replays (bit-identical to the first exec)."""
+    t = ff.Type.Z2Z if dbl else ff.Type.C2C
+    dtype = np.complex128 if dbl else np.complex64
+    for batch in (3, 8):  # 8: the XCD-aligned block order
+        x = complex_input(ora.accuracy_seed(int(t), n, batch, 11), (batch, n), dtype, "normal")
+        ref_f = ora.fft(x.astype(np.complex128), sign=ff.FORWARD)
+        ref_i = ora.fft(x.astype(np.complex128), sign=ff.INVERSE)
+        for units, p in ((6, 2), (7, 4)):
+            ff.set_tuning(units=units)
+            try:
+                plan = ff.Plan.one_d(n, t, batch)
+            finally:
+                ff.set_tuning()
+            with plan:
+                desc = plan.description()
+                if f"k_bluestein_split<{'fp64' if dbl else 'fp32'}" not in desc:
+                    continue  # no instance of this split for this M
+                assert f"k_bluestein_combine<{'fp64' if dbl else 'fp32'},{p}>" in desc
+                plan.set_stream(stream_of(gpu))
+                ex = plan.z2z if dbl else plan.c2c
+                xd = dev_tensor(x, gpu)
+                outs = []
+                for rep in range(3):
+                    yd = xd.clone().zero_()
+                    ex(xd.data_ptr(), yd.data_ptr(), ff.FORWARD if rep != 1 else ff.INVERSE)
+                    outs.append(yd)
+                sync(gpu)
+                for rep, yd in enumerate(outs):
+                    check_accuracy(host(yd), ref_i if rep == 1 else ref_f, t, n, row_len=n,
+                                   what=f"units={units} batch={batch} exec {rep}")
+                assert np.array_equal(host(outs[0]), host(outs[2]))
+                zd = xd.clone()
+                ex(zd.data_ptr(), zd.data_ptr(), ff.FORWARD)
+                sync(gpu)
+                assert np.array_equal(host(zd), host(outs[0])), "in place"
+                plan.set_graph_mode(True)
+                for rep in range(3):
+                    gd = xd.clone().zero_()
+                    ex(xd.data_ptr(), gd.data_ptr(), ff.FORWARD)
+                    sync(gpu)
+                    assert np.array_equal(host(gd), host(outs[0])), f"graph replay {rep}"
