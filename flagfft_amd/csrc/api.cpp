@@ -175,7 +175,8 @@ PlanDesc plan_desc(const Desc &d) {
   PlanDesc p;
   p.rank = d.rank;
   p.n0 = d.n[0];
-  p.n1 = d.rank == 2 ? d.n[1] : 0;
+  p.n1 = d.rank >= 2 ? d.n[1] : 0;
+  p.n2 = d.rank == 3 ? d.n[2] : 0;
   p.type = static_cast<int>(d.type);
   p.batch = d.batch;
   p.prec = d.prec;
@@ -311,6 +312,32 @@ extern "C" flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flag
   if (r != FLAGFFT_SUCCESS) return r;
   if (type != FLAGFFT_C2C || nx != 1024) return FLAGFFT_NOT_SUPPORTED;
   d.compute = compute;
+  return flagfft::build_plan(plan, d);
+}
+
+// Rank 3, contiguous batches (flagfftPlan3d / rank-3 PlanMany keep the
+// reference's FLAGFFT_NOT_SUPPORTED, src/exec/c_api_plan.cpp:274-336).
+extern "C" flagfftResult flagfftxPlan3d(flagfftHandle *plan, int nx, int ny, int nz, flagfftType type, int batch) {
+  if (plan == nullptr) return FLAGFFT_INVALID_VALUE;
+  *plan = nullptr;
+  if (batch <= 0) return FLAGFFT_INVALID_VALUE;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return FLAGFFT_INVALID_SIZE;
+  flagfft::Desc d;
+  flagfftResult r = flagfft::type_metadata(type, d);
+  if (r != FLAGFFT_SUCCESS) return r;
+  const int64_t h = nz / 2 + 1;
+  const int64_t full = static_cast<int64_t>(nx) * ny * nz, half = static_cast<int64_t>(nx) * ny * h;
+  if (static_cast<int64_t>(batch) * full * (d.real_fwd || d.real_inv ? 1 : 2) >= (int64_t{1} << 40))
+    return FLAGFFT_INVALID_SIZE;
+  d.rank = 3;
+  d.n = {nx, ny, nz};
+  d.type = type;
+  d.batch = batch;
+  d.istride = d.ostride = 1;
+  d.idist = d.real_inv ? half : full;
+  d.odist = d.real_fwd ? half : full;
+  d.inembed = {nx, ny, d.real_inv ? static_cast<int>(h) : nz};
+  d.onembed = {nx, ny, d.real_fwd ? static_cast<int>(h) : nz};
   return flagfft::build_plan(plan, d);
 }
 

@@ -1522,6 +1522,48 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
     s.gi = Geometry{1, 0, idist};
     s.go = Geometry{1, 0, odist};
     b.emit_plan1d(t.rows, s, kBufIn, kBufOut, inverse);
+  } else if (t.desc.rank == 3) {
+    // n0 x n1 x n2, n2 contiguous: rows, then the n1-point columns of every
+    // n0 slab (the 2D column form with n0 batch entries), then the n0-point
+    // columns over n1 x n2' contiguous columns
+    const int64_t n0 = t.desc.n0, n1 = t.desc.n1, n2 = t.desc.n2;
+    const bool real = t.desc.real_fwd || t.desc.real_inv;
+    const int64_t w = real ? n2 / 2 + 1 : n2;  // complex row length between the column passes
+    auto rows_of = [&](int64_t in_len, int64_t out_len) {
+      SeqSet r;
+      r.nbatch = batch * n0 * n1;
+      r.gi = Geometry{1, 0, in_len};
+      r.go = Geometry{1, 0, out_len};
+      return r;
+    };
+    SeqSet m;
+    m.nbatch = batch * n0;
+    m.ncol = w;
+    m.gi = Geometry{w, 1, n1 * w};
+    m.go = m.gi;
+    m.col_fast = true;
+    SeqSet c;
+    c.nbatch = batch;
+    c.ncol = n1 * w;
+    c.gi = Geometry{n1 * w, 1, n0 * n1 * w};
+    c.go = c.gi;
+    c.col_fast = true;
+    if (!real) {
+      b.emit_plan1d(t.rows, rows_of(n2, n2), kBufIn, kBufOut, inverse);
+      b.emit_plan1d(t.mid, m, kBufOut, kBufOut, inverse);
+      b.emit_plan1d(t.cols, c, kBufOut, kBufOut, inverse);
+    } else if (t.desc.real_fwd) {
+      // rows into a workspace (see the 2D R2C case), the column passes from it
+      const int W = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * n1 * w) * elem_bytes(t.desc.prec));
+      b.emit_plan1d(t.rows, rows_of(n2, w), kBufIn, W, false);
+      b.emit_plan1d(t.mid, m, W, W, false);
+      b.emit_plan1d(t.cols, c, W, kBufOut, false);
+    } else {
+      const int W = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * n1 * w) * elem_bytes(t.desc.prec));
+      b.emit_plan1d(t.cols, c, kBufIn, W, true);
+      b.emit_plan1d(t.mid, m, W, W, true);
+      b.emit_plan1d(t.rows, rows_of(w, n2), W, kBufOut, true);
+    }
   } else {
     const int64_t n0 = t.desc.n0, n1 = t.desc.n1, h1 = n1 / 2 + 1;
     if (!t.desc.real_fwd && !t.desc.real_inv) {

@@ -379,6 +379,28 @@ Plan1D plan_1d(int64_t n, bool real_fwd, bool real_inv, Prec p, const PlanOption
   return pl;
 }
 
+// A column super-stage reads row segments of (adjacent columns per
+// workgroup) x (element bytes).  When the single pass's widest specialised
+// tile gives segments under 64 B (a long column filling the LDS with few
+// columns), two passes of wide tiles move the data twice at full segment
+// width and win (4096-point FP32 columns: 66 us against 108 us per
+// 2049 x 4096 half spectrum).
+void widen_columns(Plan1D &cols, Prec prec, const PlanOptions &o) {
+  const bool tuned = !o.forced_passes.empty() || !o.forced_radices.empty() || o.forced_units != 0 || o.generic_kernels;
+  if (tuned || cols.core.kind != CoreKind::Leaf) return;
+  const int seg = column_segment_bytes(cols.core.radices, prec);
+  Core mc;
+  if (seg == 0 || seg >= 64 || !split_passes(cols.n, 2, pass_max(o, prec), mc.passes)) return;
+  for (int64_t r : mc.passes) {
+    std::vector<int> rad;
+    if (!radix_factor(r, rad) || column_segment_bytes(rad, prec) < 64) return;
+    mc.pass_radices.push_back(rad);
+  }
+  mc.kind = CoreKind::MultiPass;
+  mc.n = cols.n;
+  cols.core = mc;
+}
+
 }  // namespace
 
 PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
@@ -397,34 +419,16 @@ PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
   }
   if (d.rank == 1) {
     t.rows = plan_1d(d.n0, d.real_fwd, d.real_inv, d.prec, o);
-  } else {
+  } else if (d.rank == 2) {
     t.rows = plan_1d(d.n1, d.real_fwd, d.real_inv, d.prec, o);
     t.cols = plan_1d(d.n0, false, false, d.prec, o);
-    // A column super-stage reads row segments of (adjacent columns per
-    // workgroup) x (element bytes).  When the single pass's widest
-    // specialised tile gives segments under 64 B (a long column filling the
-    // LDS with few columns), two passes of wide tiles move the data twice at
-    // full segment width and win (4096-point FP32 columns: 66 us against
-    // 108 us per 2049 x 4096 half spectrum).
-    const bool tuned = !o.forced_passes.empty() || !o.forced_radices.empty() || o.forced_units != 0 ||
-                       o.generic_kernels;
-    if (!tuned && t.cols.core.kind == CoreKind::Leaf) {
-      const int seg = column_segment_bytes(t.cols.core.radices, d.prec);
-      Core mc;
-      if (seg > 0 && seg < 64 && split_passes(d.n0, 2, pass_max(o, d.prec), mc.passes)) {
-        bool wide = true;
-        for (int64_t r : mc.passes) {
-          std::vector<int> rad;
-          wide = wide && radix_factor(r, rad) && column_segment_bytes(rad, d.prec) >= 64;
-          mc.pass_radices.push_back(rad);
-        }
-        if (wide) {
-          mc.kind = CoreKind::MultiPass;
-          mc.n = d.n0;
-          t.cols.core = mc;
-        }
-      }
-    }
+    widen_columns(t.cols, d.prec, o);
+  } else {
+    t.rows = plan_1d(d.n2, d.real_fwd, d.real_inv, d.prec, o);
+    t.mid = plan_1d(d.n1, false, false, d.prec, o);
+    t.cols = plan_1d(d.n0, false, false, d.prec, o);
+    widen_columns(t.mid, d.prec, o);
+    widen_columns(t.cols, d.prec, o);
   }
   return t;
 }
@@ -433,14 +437,21 @@ std::string PlanTree::describe() const {
   std::ostringstream o;
   o << "=== FlagFFT Plan (gfx950) ===\n";
   o << "rank=" << desc.rank << " n=[" << desc.n0;
-  if (desc.rank == 2) o << "," << desc.n1;
+  if (desc.rank >= 2) o << "," << desc.n1;
+  if (desc.rank == 3) o << "," << desc.n2;
   o << "] batch=" << desc.batch << " type=" << desc.type << " precision=" << (desc.prec == Prec::F32 ? "fp32" : "fp64")
     << "\n\n-- Plan tree --\n";
   if (desc.rank == 1) {
     o << rows.describe(0) << "\n";
-  } else {
+  } else if (desc.rank == 2) {
     o << "TwoDim(n0=" << desc.n0 << ", n1=" << desc.n1 << ", order=rows-then-columns, column_pass=strided-tile)\n";
     o << "  rows (axis 1):\n" << rows.describe(4) << "\n";
+    o << "  columns (axis 0):\n" << cols.describe(4) << "\n";
+  } else {
+    o << "ThreeDim(n0=" << desc.n0 << ", n1=" << desc.n1 << ", n2=" << desc.n2
+      << ", order=rows-then-axis1-then-axis0, column_pass=strided-tile)\n";
+    o << "  rows (axis 2):\n" << rows.describe(4) << "\n";
+    o << "  columns (axis 1):\n" << mid.describe(4) << "\n";
     o << "  columns (axis 0):\n" << cols.describe(4) << "\n";
   }
   return o.str();

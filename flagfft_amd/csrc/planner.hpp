@@ -69,7 +69,7 @@ struct Plan1D {
 
 struct PlanDesc {
   int rank = 1;
-  int64_t n0 = 0, n1 = 0;     // rank 1: n0 = n
+  int64_t n0 = 0, n1 = 0, n2 = 0;  // rank 1: n0 = n; row-major n0 x n1 (x n2), last axis contiguous
   int type = 0;                // flagfftType value
   int64_t batch = 1;
   Prec prec = Prec::F32;
@@ -79,8 +79,9 @@ struct PlanDesc {
 
 struct PlanTree {
   PlanDesc desc;
-  Plan1D rows;   // rank 1: the transform; rank 2: along n1 (contiguous)
-  Plan1D cols;   // rank 2: C2C along n0 (strided)
+  Plan1D rows;   // rank 1: the transform; rank 2 / 3: along the last axis (contiguous)
+  Plan1D mid;    // rank 3: C2C along n1 (strided)
+  Plan1D cols;   // rank 2 / 3: C2C along n0 (strided)
   std::string describe() const;
 };
 

@@ -52,7 +52,7 @@ SYMBOLS = [
     "flagfftSetStream", "flagfftDestroy", "flagfftGetPlanDescription",
 ]
 EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning",
-               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute",
+               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute", "flagfftxPlan3d",
                "flagfftxSetGraphMode", "flagfftxServerCreate", "flagfftxServerPrepare", "flagfftxServerSubmit",
                "flagfftxServerPoll", "flagfftxServerWait", "flagfftxServerRelease", "flagfftxServerGetStats",
                "flagfftxServerResetStats", "flagfftxServerDestroy", "flagfftxTunedStore", "flagfftxTunedDbPath"]
@@ -125,6 +125,7 @@ def lib():
             "flagfftxPlanDescribe": [_int, _IP, _int, _int, ctypes.c_char_p, ctypes.c_size_t],
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
             "flagfftxPlan1dCompute": [_HP, _int, _int, _int, _int],
+            "flagfftxPlan3d": [_HP, _int, _int, _int, _int, _int],
             "flagfftxSetGraphMode": [_H, _int],
             "flagfftxTunedStore": [_int, _int, _int, ctypes.POINTER(Tuning), ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double],
@@ -175,6 +176,12 @@ def plan1d(nx: int, type_: int, batch: int):
 def plan1d_compute(nx: int, type_: int, batch: int, compute: int):
     h = ctypes.c_void_p()
     r = lib().flagfftxPlan1dCompute(ctypes.byref(h), nx, int(type_), batch, int(compute))
+    return r, h
+
+
+def plan3d_ext(nx: int, ny: int, nz: int, type_: int, batch: int = 1):
+    h = ctypes.c_void_p()
+    r = lib().flagfftxPlan3d(ctypes.byref(h), nx, ny, nz, int(type_), batch)
     return r, h
 
 
@@ -338,6 +345,12 @@ class Plan:
     def two_d(cls, nx: int, ny: int, type_: int):
         r, h = plan2d(nx, ny, type_)
         check("flagfftPlan2d", r)
+        return cls(h)
+
+    @classmethod
+    def three_d(cls, nx: int, ny: int, nz: int, type_: int, batch: int = 1):
+        r, h = plan3d_ext(nx, ny, nz, type_, batch)
+        check("flagfftxPlan3d", r)
         return cls(h)
 
     @classmethod

@@ -102,6 +102,17 @@ typedef enum flagfftxCompute_t {
  * usual calls. */
 flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType type, int batch, int compute);
 
+/* Rank-3 transforms of `batch` contiguous nx x ny x nz arrays (row-major, nz
+ * contiguous; real transforms use nx x ny x (nz/2+1) complex on the
+ * spectrum side), executed with the usual flagfftExec* calls: rows along nz,
+ * then the ny-point and nx-point columns in place of transposes.  No
+ * counterpart in the reference, whose flagfftPlan3d and rank-3
+ * flagfftPlanMany return FLAGFFT_NOT_SUPPORTED (ctest/test_plan.cpp:151-159);
+ * those two keep that answer here.  Errors: null plan or batch <= 0 ->
+ * FLAGFFT_INVALID_VALUE, a size <= 0 -> FLAGFFT_INVALID_SIZE, bad type ->
+ * FLAGFFT_INVALID_TYPE, no device -> FLAGFFT_INVALID_DEVICE. */
+flagfftResult flagfftxPlan3d(flagfftHandle *plan, int nx, int ny, int nz, flagfftType type, int batch);
+
 /* ---- serving loop ---------------------------------------------------------
  * A server owns a device arena of `hbm_budget_bytes` for request outputs and
  * workspaces, a cache of plans and twiddle tables resident on the device, and

@@ -209,6 +209,29 @@ def irfft2(X, n1: int, nthreads: int = 0) -> np.ndarray:
     return out
 
 
+def _along(x, axis: int, f):
+    """Apply a last-axis transform f along `axis` (FP64, contiguous copies)."""
+    y = f(np.ascontiguousarray(np.moveaxis(x, axis, -1)))
+    return np.ascontiguousarray(np.moveaxis(y, -1, axis))
+
+
+def fft3(x, sign: int = -1, nthreads: int = 0) -> np.ndarray:
+    """C2C over the last three axes by separability: the 2D transform of the
+    last two axes, then the 1D transform along axis -3."""
+    return _along(fft2(x, sign, nthreads), -3, lambda v: fft(v, sign, nthreads))
+
+
+def rfft3(x, nthreads: int = 0) -> np.ndarray:
+    """R2C over the last three axes: rfft2 of the last two, then C2C along axis -3."""
+    return _along(rfft2(x, nthreads), -3, lambda v: fft(v, -1, nthreads))
+
+
+def irfft3(X, n2: int, nthreads: int = 0) -> np.ndarray:
+    """Unnormalised C2R over the last three axes (Im of the DC / Nyquist bins
+    of the last axis ignored): inverse C2C along axis -3, then irfft2."""
+    return irfft2(_along(_c128(X), -3, lambda v: fft(v, 1, nthreads)), n2, nthreads)
+
+
 def primitive_root(p: int) -> int:
     return int(lib().ora_primitive_root(p))
 

@@ -92,6 +92,23 @@ def test_valid_descriptors_reach_the_device_check():
     assert ff.plan2d(64, 32, T.R2C)[0] == R.INVALID_DEVICE
 
 
+def test_rank3_extension_validation():
+    """flagfftxPlan3d argument checks; flagfftPlan3d itself keeps FLAGFFT_NOT_SUPPORTED."""
+    import torch
+
+    assert ff.lib().flagfftxPlan3d(None, 8, 8, 8, int(T.C2C), 1) == R.INVALID_VALUE
+    assert ff.plan3d_ext(8, 8, 8, T.C2C, 0)[0] == R.INVALID_VALUE
+    assert ff.plan3d_ext(0, 8, 8, T.C2C, 1)[0] == R.INVALID_SIZE
+    assert ff.plan3d_ext(8, 8, -1, T.R2C, 1)[0] == R.INVALID_SIZE
+    assert ff.plan3d_ext(8, 8, 8, 0x77, 1)[0] == R.INVALID_TYPE
+    r, h = ff.plan3d(32, 16, 8, T.C2C)
+    assert r == R.NOT_SUPPORTED and not h.value
+    if not torch.cuda.is_available():
+        for t in ALL_TYPES:
+            r, h = ff.plan3d_ext(16, 12, 10, t, 2)
+            assert r == R.INVALID_DEVICE and not h.value
+
+
 def test_compute_mode_validation():
     """flagfftxPlan1dCompute: unknown mode, unsupported type / length, device check."""
     import torch

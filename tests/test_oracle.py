@@ -192,6 +192,25 @@ def test_2d_separability():
         assert rel(ora.irfft2(ora.rfft2(xr), n1), n0 * n1 * xr) < 1e-14
 
 
+def test_3d_compositions():
+    """fft3 / rfft3 / irfft3 against numpy's pocketfft (a library routine) and
+    the closed form of a shifted delta; a transposed axis order would fail
+    the non-cubic shapes."""
+    for n0, n1, n2 in [(4, 6, 10), (7, 5, 3), (2, 9, 8)]:
+        x = ora.complex_signed_unit(n0 * n1 * n2 + 1, (2, n0, n1, n2))
+        assert rel(ora.fft3(x), np.fft.fftn(x, axes=(1, 2, 3))) < 1e-14
+        assert rel(ora.fft3(x, sign=1), np.fft.ifftn(x, axes=(1, 2, 3)) * n0 * n1 * n2) < 1e-14
+        xr = x.real.copy()
+        assert rel(ora.rfft3(xr), np.fft.rfftn(xr, axes=(1, 2, 3))) < 1e-14
+        assert rel(ora.irfft3(ora.rfft3(xr), n2), n0 * n1 * n2 * xr) < 1e-14
+        d = np.zeros((n0, n1, n2), complex)
+        a, b, c = 1, 2 % n1, 3 % n2
+        d[a, b, c] = 1
+        k0, k1, k2 = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+        wave = np.exp(-2j * np.pi * (k0 * a / n0 + k1 * b / n1 + k2 * c / n2))
+        assert rel(ora.fft3(d), wave) < 1e-14
+
+
 # ---- Rader / Bluestein restatements --------------------------------------------
 def _brute_primitive_root(p):
     for g in range(2, p):
