@@ -56,19 +56,25 @@ class Case:
         rdt = torch.float64 if dbl else torch.float32
         esz = 16 if dbl else 8
         T = ff.Type
-        if isinstance(n, tuple):  # 2D (n0 x n1, row-major), one transform per exec
-            n0, n1 = n
-            h = n1 // 2 + 1
+        if isinstance(n, tuple):  # 2D / 3D (row-major, last axis contiguous), one transform per exec
+            lead, nl = n[:-1], n[-1]
+            h = nl // 2 + 1
+            outer = 1
+            for v in lead:
+                outer *= v
             self.batch = 1
+
+            def plan(t):
+                return ff.Plan.two_d(n[0], n[1], t) if len(n) == 2 else ff.Plan.three_d(n[0], n[1], n[2], t, 1)
+
             if kind == "c2c":
-                self.plans = [ff.Plan.two_d(n0, n1, T.Z2Z if dbl else T.C2C)]
-                self.alg_bytes = n0 * n1 * esz * 2
-                self.shapes = [((n0, n1), cdt), ((n0, n1), cdt)]
+                self.plans = [plan(T.Z2Z if dbl else T.C2C)]
+                self.alg_bytes = outer * nl * esz * 2
+                self.shapes = [(n, cdt), (n, cdt)]
             else:
-                self.plans = [ff.Plan.two_d(n0, n1, T.D2Z if dbl else T.R2C),
-                              ff.Plan.two_d(n0, n1, T.Z2D if dbl else T.C2R)]
-                self.alg_bytes = 2 * (n0 * n1 * esz // 2 + n0 * h * esz)
-                self.shapes = [((n0, n1), rdt), ((n0, h), cdt), ((n0, n1), rdt)]
+                self.plans = [plan(T.D2Z if dbl else T.R2C), plan(T.Z2D if dbl else T.C2R)]
+                self.alg_bytes = 2 * (outer * nl * esz // 2 + outer * h * esz)
+                self.shapes = [(n, rdt), (lead + (h,), cdt), (n, rdt)]
             self.sets = []
             return
         h = n // 2 + 1
@@ -155,6 +161,8 @@ SUITE = [  # the other configurations, reported with --suite (not the headline l
     [("R2C->C2R f32 N=2^20 x16", "real", 1 << 20, 16, False)],
     [("2D R2C->C2R f32 4096x4096", "real", (4096, 4096), 1, False)],
     [("2D Z2Z f64 2048x2048", "c2c", (2048, 2048), 1, True)],
+    [("3D C2C f32 256x256x256", "c2c", (256, 256, 256), 1, False)],
+    [("3D R2C->C2R f32 256x256x256", "real", (256, 256, 256), 1, False)],
 ]
 
 
@@ -473,10 +481,11 @@ def accuracy_sample(case, torch, rows: int = 64):
     b = case.sets[0]
     case.run(0)
     torch.cuda.synchronize()
-    if isinstance(case.n, tuple):  # 2D: the whole transform
+    if isinstance(case.n, tuple):  # 2D / 3D: the whole transform
         x = b[0].cpu().numpy().astype(np.complex128)
         y = b[1].cpu().numpy()
-        return float(f"{ora.rel_l2(y.reshape(1, -1), ora.fft2(x[None]).reshape(1, -1))[0]:.3e}")
+        ref = ora.fft2(x[None]) if len(case.n) == 2 else ora.fft3(x[None])
+        return float(f"{ora.rel_l2(y.reshape(1, -1), ref.reshape(1, -1))[0]:.3e}")
     x = b[0][:rows].cpu().numpy().astype(np.complex128)
     y = b[1][:rows].cpu().numpy()
     return float(f"{ora.rel_l2(y, ora.fft(x))[0]:.3e}")

@@ -21,9 +21,11 @@ passes move, and the effective GB/s of both libraries; --check adds the
 relative L2 distance between the two outputs.
 
 Differences on purpose: real transforms of rank 2 are accepted (batch 1, the
-reference's CLI refuses them); 1D real in-place cases use the padded layout
-of 2(N/2+1) reals per row through PlanMany, the in-place layout the library
-accepts (src/exec/c_api_internal.cpp:74-89).
+reference's CLI refuses them); rank 3 runs through flagfftxPlan3d (any type,
+batched; the reference's library answers FLAGFFT_NOT_SUPPORTED there); 1D
+real in-place cases use the padded layout of 2(N/2+1) reals per row through
+PlanMany, the in-place layout the library accepts
+(src/exec/c_api_internal.cpp:74-89).
 
 tune runs the autotuner (flagfft_amd.autotune) with the remaining arguments;
 the reference's tune command is a placeholder (src/cli_tools/tune/tune.cpp:7-9).
@@ -51,7 +53,7 @@ Bench options:
                            rank 1: --shape 16,32,64
                            rank 2: --shape 23x42,128x64
   --api c2c|z2z|r2c|d2z|c2r|z2d   FFT type (default: c2c)
-  --batch N                Batch size for rank 1 and complex rank 2 (default: 1)
+  --batch N                Batch size for rank 1, complex rank 2 and rank 3 (default: 1)
   --direction forward|inverse       (default: forward)
   --placement out-of-place|in-place (default: out-of-place)
   --warmup N               Warmup iterations (default: 10)
@@ -183,8 +185,6 @@ def parse_args(argv: list[str]) -> dict:
             fail("--shape dimension count does not match --rank")
         if args["rank"] == 2 and api not in COMPLEX and args["batch"] != 1:
             fail("--batch is only supported with --rank 1 or complex --rank 2")
-        if args["rank"] > 2 and args["batch"] != 1:
-            fail("--batch is only supported with --rank 1 or complex --rank 2")
     if api in REAL_FWD and args["direction"] != "forward":
         fail(f"{api} only supports forward direction")
     if api in REAL_INV and args["direction"] != "inverse":
@@ -250,10 +250,13 @@ class Case:
             self.plan = ff.Plan.two_d(shape[0], shape[1], self.type)
             self.in_pitch = n if api in REAL_FWD else padded
             self.out_pitch = padded if api in REAL_FWD else n
-        else:
-            r, h = ff.plan3d(shape[0], shape[1], shape[2], self.type)
-            ff.check("create FlagFFT plan", r)
-            raise CliError("rank 3 plan unexpectedly created", EXIT_RUNTIME_ERROR)
+        else:  # rank 3 through the extension (flagfftPlan3d answers NOT_SUPPORTED, as in the reference)
+            self.plan = ff.Plan.three_d(shape[0], shape[1], shape[2], self.type, batch)
+            if api in COMPLEX:
+                self.in_pitch = self.out_pitch = 2 * n
+            else:
+                self.in_pitch = n if api in REAL_FWD else padded
+                self.out_pitch = padded if api in REAL_FWD else n
         fn = {"c2c": self.plan.c2c, "z2z": self.plan.z2z, "r2c": self.plan.r2c, "d2z": self.plan.d2z,
               "c2r": self.plan.c2r, "z2d": self.plan.z2d}[api]
         d = ff.INVERSE if spec["direction"] == "inverse" else ff.FORWARD

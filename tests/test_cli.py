@@ -2,8 +2,8 @@
 
 CPU: argument validation and exit codes (src/cli_tools/common/cli_utils.hpp:14-17).
 GPU: JSON report shape (src/cli_tools/bench/report.cpp:42-67), agreement with
-the torch.fft comparator for every API and placement, rank 3 refused at plan
-time with the runtime-error exit code.
+the torch.fft comparator for every API, placement and rank (rank 3 through
+flagfftxPlan3d), library errors mapped to the runtime-error exit code.
 """
 import json
 
@@ -68,6 +68,10 @@ GPU_CASES = [
     (["--rank", "2", "--shape", "128x256", "--api", "d2z"], "d2z"),
     (["--rank", "2", "--shape", "256x130", "--api", "r2c", "--placement", "in-place"], "r2c"),
     (["--rank", "2", "--shape", "64x66", "--api", "c2r", "--direction", "inverse", "--placement", "in-place"], "c2r"),
+    (["--rank", "3", "--shape", "16x12x10,9x8x7", "--batch", "2"], "c2c"),
+    (["--rank", "3", "--shape", "32x16x20", "--api", "d2z"], "d2z"),
+    (["--rank", "3", "--shape", "12x10x9", "--api", "c2r", "--direction", "inverse", "--placement", "in-place"],
+     "c2r"),
 ]
 
 
@@ -93,8 +97,15 @@ def test_bench_json_matches_comparator(gpu, args, api, capsys):
 
 
 @pytest.mark.gpu
-def test_rank3_is_runtime_error(gpu, capsys):
-    assert cli.main(["bench", "--rank", "3", "--shape", "8x8x8", "--no-ref"]) == cli.EXIT_RUNTIME_ERROR
+def test_library_errors_are_runtime_errors(gpu, capsys, monkeypatch):
+    """A plan the library refuses ends with exit code 2 (cli_utils.cpp check_flagfft)."""
+    from flagfft_amd import flagfft as ff
+
+    def refuse(*args, **kw):
+        raise ff.FlagFFTError("flagfftPlan1d", int(ff.Result.NOT_SUPPORTED))
+
+    monkeypatch.setattr(ff.Plan, "one_d", refuse)
+    assert cli.main(["bench", "--rank", "1", "--shape", "64", "--no-ref"]) == cli.EXIT_RUNTIME_ERROR
     assert "NOT_SUPPORTED" in capsys.readouterr().err
 
 
