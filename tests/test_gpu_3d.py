@@ -106,6 +106,39 @@ def test_real_3d_in_place(gpu, shape):
     check_accuracy(z, x.astype(np.float64) * n, ff.Type.C2R, n, row_len=n, factor=2.0, what="in-place roundtrip")
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 8, 1), (7, 1, 5), (2, 1, 1), (1, 1, 16), (3, 2, 2), (1, 5, 2)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_degenerate_axes(gpu, shape):
+    """Axes of length 1 and 2 (length-1 transforms are the identity) for C2C, Z2Z, R2C and C2R."""
+    n0, n1, n2 = shape
+    n = n0 * n1 * n2
+    h = n2 // 2 + 1
+    x = complex_input(n + 101, (1, n0, n1, n2), np.complex128)
+    for t, dtype in ((ff.Type.C2C, np.complex64), (ff.Type.Z2Z, np.complex128)):
+        xd = dev_tensor(x.astype(dtype), gpu)
+        yd = xd.clone().zero_()
+        with ff.Plan.three_d(n0, n1, n2, t) as p:
+            p.set_stream(stream_of(gpu))
+            (p.z2z if t == ff.Type.Z2Z else p.c2c)(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
+            sync(gpu)
+        check_accuracy(host(yd), ora.fft3(x.astype(dtype).astype(np.complex128)), t, max(n, 2), row_len=n,
+                       factor=2.0, what="degenerate C2C")
+    xr = x.real.astype(np.float32)
+    xd = dev_tensor(xr, gpu)
+    Xd = dev_tensor(np.zeros((1, n0, n1, h), np.complex64), gpu)
+    zd = xd.clone().zero_()
+    with ff.Plan.three_d(n0, n1, n2, ff.Type.R2C) as pf, ff.Plan.three_d(n0, n1, n2, ff.Type.C2R) as pi:
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        pf.r2c(xd.data_ptr(), Xd.data_ptr())
+        pi.c2r(Xd.data_ptr(), zd.data_ptr())
+        sync(gpu)
+    check_accuracy(host(Xd), ora.rfft3(xr.astype(np.float64)), ff.Type.R2C, max(n, 2), row_len=n0 * n1 * h,
+                   factor=2.0, what="degenerate R2C")
+    check_accuracy(host(zd), xr.astype(np.float64) * n, ff.Type.C2R, max(n, 2), row_len=n, factor=2.0,
+                   what="degenerate roundtrip")
+
+
 def test_c2c_3d_in_place_and_graph(gpu):
     n0, n1, n2 = 32, 24, 40
     n = n0 * n1 * n2
