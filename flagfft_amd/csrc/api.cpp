@@ -37,6 +37,8 @@ struct Desc {
   Prec prec = Prec::F32;
   bool real_fwd = false, real_inv = false;
   int compute = 0;
+  bool general = false;  // a C2C/Z2Z layout outside the reference's accepted ones (flagfftxPlanMany)
+  Strides strides;
 };
 
 struct PlanImpl {
@@ -97,7 +99,7 @@ TuneKey tune_key(flagfftType type, int64_t n, int64_t batch) {
 // directions), else the defaults.  `source` names the database row used.
 PlanOptions plan_options(const Desc &d, std::string &source) {
   source.clear();
-  if (t_explicit || d.rank != 1 || d.compute != 0) return t_options;
+  if (t_explicit || d.rank != 1 || d.compute != 0 || d.general) return t_options;
   std::string js;
   if (!tuned_lookup(tune_key(d.type, d.n[0], d.batch), js)) return t_options;
   TuneOverrides t;
@@ -211,20 +213,26 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
     std::string tuned;
     const PlanOptions opts = plan_options(d, tuned);
     impl->tree = plan_tree(plan_desc(d), opts);
+    const Strides *st = d.general ? &d.strides : nullptr;
     if (d.real_fwd) {
-      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist);
+      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist, true, st);
     } else if (d.real_inv) {
-      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist);
+      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist, true, st);
     } else {
-      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist);
-      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist);
+      impl->fwd = build_executable(impl->tree, false, opts, d.idist, d.odist, true, st);
+      impl->inv = build_executable(impl->tree, true, opts, d.idist, d.odist, true, st);
     }
     PlanOptions ip = opts;
     ip.in_place = true;
-    if (impl->fwd && impl->fwd->inplace_hazard) impl->fwd_ip = build_executable(impl->tree, false, ip, d.idist, d.odist);
-    if (impl->inv && impl->inv->inplace_hazard) impl->inv_ip = build_executable(impl->tree, true, ip, d.idist, d.odist);
+    if (impl->fwd && impl->fwd->inplace_hazard)
+      impl->fwd_ip = build_executable(impl->tree, false, ip, d.idist, d.odist, true, st);
+    if (impl->inv && impl->inv->inplace_hazard)
+      impl->inv_ip = build_executable(impl->tree, true, ip, d.idist, d.odist, true, st);
     std::ostringstream o;
     o << impl->tree.describe();
+    if (d.general)
+      o << "layout: istride=" << d.istride << " idist=" << d.idist << " input row=" << d.strides.irow
+        << " ostride=" << d.ostride << " odist=" << d.odist << " output row=" << d.strides.orow << "\n";
     if (!tuned.empty()) o << "tuned-plan: " << tuned << "\n";
     o << "\n-- Forward execution --\n" << (impl->fwd ? impl->fwd->describe() : std::string("(not compiled)\n"));
     if (impl->inv) o << "\n-- Inverse execution --\n" << impl->inv->describe();
@@ -312,6 +320,30 @@ extern "C" flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flag
   if (r != FLAGFFT_SUCCESS) return r;
   if (type != FLAGFFT_C2C || nx != 1024) return FLAGFFT_NOT_SUPPORTED;
   d.compute = compute;
+  return flagfft::build_plan(plan, d);
+}
+
+// flagfftPlanMany with the advanced layouts the reference answers
+// FLAGFFT_NOT_SUPPORTED for (src/exec/c_api_internal.cpp:60-120): any
+// positive element stride, embeds and distances for C2C / Z2Z of rank 1 and 2.
+extern "C" flagfftResult flagfftxPlanMany(flagfftHandle *plan, int rank, int *n, int *inembed, int istride, int idist,
+                                         int *onembed, int ostride, int odist, flagfftType type, int batch) {
+  if (plan == nullptr) return FLAGFFT_INVALID_VALUE;
+  *plan = nullptr;
+  flagfft::Desc d;
+  flagfftResult r = flagfft::make_desc(rank, n, inembed, istride, idist, onembed, ostride, odist, type, batch, d);
+  if (r == FLAGFFT_SUCCESS) return flagfft::build_plan(plan, d);
+  if (r != FLAGFFT_NOT_SUPPORTED) return r;
+  if ((type != FLAGFFT_C2C && type != FLAGFFT_Z2Z) || rank > 2) return FLAGFFT_NOT_SUPPORTED;
+  if (istride <= 0 || ostride <= 0 || idist <= 0 || odist <= 0) return FLAGFFT_INVALID_VALUE;
+  const int64_t last = d.n[static_cast<size_t>(rank - 1)];
+  const int64_t iemb = rank == 2 ? d.inembed[1] : last, oemb = rank == 2 ? d.onembed[1] : last;
+  if (iemb < last || oemb < last) return FLAGFFT_INVALID_VALUE;
+  d.general = true;
+  d.strides.istride = istride;
+  d.strides.ostride = ostride;
+  d.strides.irow = iemb * istride;
+  d.strides.orow = oemb * ostride;
   return flagfft::build_plan(plan, d);
 }
 

@@ -1511,11 +1511,31 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
 }
 
 std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o, int64_t idist,
-                                             int64_t odist, bool allocate_ws) {
+                                             int64_t odist, bool allocate_ws, const Strides *st) {
   auto ex = std::make_unique<Executable>();
   Builder b(t.desc.prec, o, *ex, nullptr);
   const int64_t batch = t.desc.batch;
-  if (t.desc.rank == 1) {
+  if (st != nullptr) {
+    // general complex layouts: the run-time-radix and LDS kernels take any
+    // element / row / batch stride; the specialised kernels take the
+    // contiguous cases only
+    if (t.desc.real_fwd || t.desc.real_inv || t.desc.rank > 2) throw std::runtime_error("strided layouts: C2C/Z2Z rank 1-2");
+    SeqSet r;
+    r.nbatch = batch;
+    r.ncol = t.desc.rank == 2 ? t.desc.n0 : 1;
+    r.gi = Geometry{st->istride, st->irow, idist};
+    r.go = Geometry{st->ostride, st->orow, odist};
+    b.emit_plan1d(t.rows, r, kBufIn, kBufOut, inverse);
+    if (t.desc.rank == 2) {
+      SeqSet c;
+      c.nbatch = batch;
+      c.ncol = t.desc.n1;
+      c.gi = Geometry{st->orow, st->ostride, odist};
+      c.go = c.gi;
+      c.col_fast = true;
+      b.emit_plan1d(t.cols, c, kBufOut, kBufOut, inverse);
+    }
+  } else if (t.desc.rank == 1) {
     SeqSet s;
     s.nbatch = batch;
     s.ncol = 1;

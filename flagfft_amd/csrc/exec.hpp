@@ -208,9 +208,17 @@ class Builder {
                    int64_t R);
 };
 
+// General complex layout of rank 1 / 2 (advanced PlanMany): element y of
+// row x of batch entry b at b * dist + x * row + y * stride (row = embed[1] *
+// stride; rank 1 has one row).
+struct Strides {
+  int64_t istride = 1, ostride = 1, irow = 0, orow = 0;
+};
+
 // Whole-plan builders (forward / inverse executables of a PlanTree).
 // allocate_ws = false leaves the workspaces to the caller (Executable::run_batch).
 std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, const PlanOptions &o,
-                                             int64_t idist, int64_t odist, bool allocate_ws = true);
+                                             int64_t idist, int64_t odist, bool allocate_ws = true,
+                                             const Strides *strides = nullptr);
 
 }  // namespace flagfft

@@ -52,7 +52,7 @@ SYMBOLS = [
     "flagfftSetStream", "flagfftDestroy", "flagfftGetPlanDescription",
 ]
 EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "flagfftxSetTuning",
-               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute", "flagfftxPlan3d",
+               "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute", "flagfftxPlan3d", "flagfftxPlanMany",
                "flagfftxSetGraphMode", "flagfftxServerCreate", "flagfftxServerPrepare", "flagfftxServerSubmit",
                "flagfftxServerPoll", "flagfftxServerWait", "flagfftxServerRelease", "flagfftxServerGetStats",
                "flagfftxServerResetStats", "flagfftxServerDestroy", "flagfftxTunedStore", "flagfftxTunedDbPath"]
@@ -126,6 +126,7 @@ def lib():
             "flagfftxPlanStats": [_H, _int, _IP, ctypes.POINTER(ctypes.c_longlong)],
             "flagfftxPlan1dCompute": [_HP, _int, _int, _int, _int],
             "flagfftxPlan3d": [_HP, _int, _int, _int, _int, _int],
+            "flagfftxPlanMany": [_HP, _int, _IP, _IP, _int, _int, _IP, _int, _int, _int, _int],
             "flagfftxSetGraphMode": [_H, _int],
             "flagfftxTunedStore": [_int, _int, _int, ctypes.POINTER(Tuning), ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double],
@@ -201,6 +202,13 @@ def plan_many(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, 
     h = ctypes.c_void_p()
     r = lib().flagfftPlanMany(ctypes.byref(h), rank, _arr(n), _arr(inembed), istride, idist, _arr(onembed), ostride,
                               odist, int(type_), batch)
+    return r, h
+
+
+def plan_many_ext(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch):
+    h = ctypes.c_void_p()
+    r = lib().flagfftxPlanMany(ctypes.byref(h), rank, _arr(n), _arr(inembed), istride, idist, _arr(onembed), ostride,
+                               odist, int(type_), batch)
     return r, h
 
 
@@ -351,6 +359,12 @@ class Plan:
     def three_d(cls, nx: int, ny: int, nz: int, type_: int, batch: int = 1):
         r, h = plan3d_ext(nx, ny, nz, type_, batch)
         check("flagfftxPlan3d", r)
+        return cls(h)
+
+    @classmethod
+    def many_ext(cls, rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch):
+        r, h = plan_many_ext(rank, n, inembed, istride, idist, onembed, ostride, odist, type_, batch)
+        check("flagfftxPlanMany", r)
         return cls(h)
 
     @classmethod

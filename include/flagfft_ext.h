@@ -113,6 +113,19 @@ flagfftResult flagfftxPlan1dCompute(flagfftHandle *plan, int nx, flagfftType typ
  * FLAGFFT_INVALID_TYPE, no device -> FLAGFFT_INVALID_DEVICE. */
 flagfftResult flagfftxPlan3d(flagfftHandle *plan, int nx, int ny, int nz, flagfftType type, int batch);
 
+/* flagfftPlanMany with the advanced layouts as well: for FLAGFFT_C2C /
+ * FLAGFFT_Z2Z of rank 1 or 2, any element stride >= 1, distance >= 1 and
+ * embed (embed[rank-1] >= n[rank-1]; element y of row x of batch entry b at
+ * b * dist + (x * embed[1] + y) * stride), which the reference answers with
+ * FLAGFFT_NOT_SUPPORTED (src/exec/c_api_internal.cpp:60-120; flagfftPlanMany
+ * keeps that answer here).  Layouts flagfftPlanMany accepts behave exactly as
+ * there; other real layouts and rank 3 stay FLAGFFT_NOT_SUPPORTED; a stride,
+ * distance or embed out of range is FLAGFFT_INVALID_VALUE.  No counterpart in
+ * the reference.  The output layout must not overlap the input unless both
+ * are the same layout in the same buffer. */
+flagfftResult flagfftxPlanMany(flagfftHandle *plan, int rank, int *n, int *inembed, int istride, int idist,
+                               int *onembed, int ostride, int odist, flagfftType type, int batch);
+
 /* ---- serving loop ---------------------------------------------------------
  * A server owns a device arena of `hbm_budget_bytes` for request outputs and
  * workspaces, a cache of plans and twiddle tables resident on the device, and

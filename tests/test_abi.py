@@ -109,6 +109,26 @@ def test_rank3_extension_validation():
             assert r == R.INVALID_DEVICE and not h.value
 
 
+def test_advanced_layout_extension_validation():
+    """flagfftxPlanMany: general C2C / Z2Z layouts of rank 1-2 pass validation; flagfftPlanMany keeps
+    NOT_SUPPORTED for them (ctest/test_plan.cpp:105-124); real layouts and rank 3 stay unsupported."""
+    import torch
+
+    strided = (2, [64, 32], [64, 35], 2, 64 * 35 * 2, [64, 40], 1, 64 * 40, T.C2C, 2)
+    assert ff.plan_many(*strided)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many_ext(1, [256], None, 1, 256, None, 1, 256, 0x77, 1)[0] == R.INVALID_TYPE
+    assert ff.plan_many_ext(1, [0], None, 1, 256, None, 1, 256, T.C2C, 1)[0] == R.INVALID_SIZE
+    assert ff.plan_many_ext(1, [256], None, 0, 256, None, 1, 256, T.C2C, 1)[0] == R.INVALID_VALUE
+    assert ff.plan_many_ext(1, [256], None, 2, 0, None, 1, 256, T.C2C, 1)[0] == R.INVALID_VALUE
+    assert ff.plan_many_ext(2, [64, 32], [64, 31], 1, 4096, None, 1, 2048, T.C2C, 1)[0] == R.INVALID_VALUE
+    assert ff.plan_many_ext(1, [256], None, 2, 600, None, 1, 256, T.R2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many_ext(3, [8, 8, 8], None, 2, 2048, None, 1, 512, T.C2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.lib().flagfftxPlanMany(None, 1, None, None, 1, 1, None, 1, 1, int(T.C2C), 1) == R.INVALID_VALUE
+    if not torch.cuda.is_available():
+        assert ff.plan_many_ext(*strided)[0] == R.INVALID_DEVICE
+        assert ff.plan_many_ext(1, [256], None, 3, 800, None, 2, 600, T.Z2Z, 4)[0] == R.INVALID_DEVICE
+
+
 def test_compute_mode_validation():
     """flagfftxPlan1dCompute: unknown mode, unsupported type / length, device check."""
     import torch
