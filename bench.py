@@ -10,10 +10,11 @@ the timed region reads HBM.
 
 value          algorithmic HBM GB/s of the whole step: sum over transforms of
                batch * (input bytes + output bytes), divided by wall time; the
-               transform families (independent) are spread over one HIP stream
-               per hardware queue (4 per process), longest-first by their
-               device time alone; config.single_stream_gbps is the same step
-               on one stream
+               transform families (independent) are spread longest-first by
+               their device time alone over 1 to 4 HIP streams (4 hardware
+               queues per process), the count chosen by a short calibration
+               region before the timed one; config.single_stream_gbps is the
+               same step on one stream
 roofline       dominant kernel (by total device time): algorithmic bytes per
                launch / mean launch duration from HIP events recorded on the
                plan's stream around every launch of a second timed region of
@@ -211,13 +212,48 @@ def assign_streams(cases, steps, torch, stream, nq):
             e1.record(stream)
         e1.synchronize()
         times.append(e0.elapsed_time(e1) / steps)
+    return lpt(times, nq), times
+
+
+def lpt(times, nq):
+    """Longest-first assignment of jobs with the given times to nq streams."""
     load = [0.0] * nq
-    slot = [0] * len(cases)
-    for i in sorted(range(len(cases)), key=lambda i: -times[i]):
+    slot = [0] * len(times)
+    for i in sorted(range(len(times)), key=lambda i: -times[i]):
         q = min(range(nq), key=lambda k: load[k])
         slot[i] = q
         load[q] += times[i]
-    return slot, times
+    return slot
+
+
+def pick_streams(cases, times, torch, dev, nq_max, steps=40):
+    """Number of streams for the families: every count from 1 to nq_max with the
+    longest-first assignment, timed over `steps` steps (best of two regions);
+    the fastest count wins.  Returns (streams, slot, {count: us per step})."""
+    table = {}
+    best = None
+    for nq in range(1, nq_max + 1):
+        streams = [torch.cuda.Stream(device=dev) for _ in range(nq)]
+        slot = lpt(times, nq)
+        for i, c in enumerate(cases):
+            c.stream = streams[slot[i]]
+            c.bind(c.stream.cuda_stream)
+        for s in range(3):
+            for c in cases:
+                c.run(s)
+        t = float("inf")
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(steps):
+                for c in cases:
+                    c.run(s)
+            torch.cuda.synchronize()
+            t = min(t, time.perf_counter() - t0)
+        table[nq] = round(t / steps * 1e6, 1)
+        if best is None or t < best[0]:
+            best = (t, streams, slot)
+    return best[1], best[2], table
 
 
 def measure(cases, steps, warmup, torch, ff, stream, case_streams, slot):
@@ -234,20 +270,19 @@ def measure(cases, steps, warmup, torch, ff, stream, case_streams, slot):
             c.bind(st.cuda_stream)
 
     def region(first_step):
+        # each plan is bound to its stream (flagfftSetStream): no torch stream context needed
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(steps):
             for c in cases:
-                with torch.cuda.stream(c.stream):
-                    c.run(first_step + s)
+                c.run(first_step + s)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
     bind(True)
     for s in range(warmup):
         for c in cases:
-            with torch.cuda.stream(c.stream):
-                c.run(s)
+            c.run(s)
     t_multi = region(warmup)
     bind(False)
     for c in cases:
@@ -501,8 +536,11 @@ def run_workload(spec, args, torch, ff, dev, stream, seed):
         c.allocate(nsets, seed + i)
     torch.cuda.synchronize()
     nq = min(hw_queues(), len(cases))
-    case_streams = [torch.cuda.Stream(device=dev) for _ in range(nq)]
     slot, alone_ms = assign_streams(cases, max(args.warmup, 3), torch, stream, nq)
+    if nq > 1:
+        case_streams, slot, stream_table = pick_streams(cases, alone_ms, torch, dev, nq)
+    else:
+        case_streams, stream_table = [torch.cuda.Stream(device=dev)], {}
     t, t1, launches, per_case = measure(cases, args.steps, args.warmup, torch, ff, stream, case_streams, slot)
     alg = sum(c.alg_bytes for c in cases)
     value = args.steps * alg / t / 1e9
@@ -510,6 +548,7 @@ def run_workload(spec, args, torch, ff, dev, stream, seed):
     pass_bytes = sum(nb * calls for (ms, calls, nb) in launches.values())
     return cases, {"value": value, "ms_per_step": t / args.steps * 1e3, "alg_bytes_per_step": alg,
                    "single_stream_gbps": args.steps * alg / t1 / 1e9, "streams": len(case_streams),
+                   "stream_calibration_us_per_step": stream_table,
                    "pass_gbps": pass_bytes / t / 1e9, "roofline": roof, "kernels": kernels,
                    "per_case": per_case, "buffer_sets": nsets, "footprint_mb": round(nsets * footprint / 2**20)}
 
@@ -585,7 +624,8 @@ def main():
             "pass_bytes_gbps": round(res["pass_gbps"], 2),
             "input_rotation": f"{res['buffer_sets']} buffer sets, {res['footprint_mb']} MiB",
             "parallelism": f"single GPU, transform families over {res['streams']} HIP streams "
-                           f"(one per hardware queue), balanced by device time",
+                           f"(count calibrated over 1-{hw_queues()}), balanced by device time",
+            "stream_calibration_us_per_step": res["stream_calibration_us_per_step"],
             "single_stream_gbps": round(res["single_stream_gbps"], 2),
         },
         "roofline": res["roofline"],
