@@ -51,6 +51,14 @@ def test_multipass_and_bluestein_sizes(gpu, n, dtype):
         _case(gpu, n, batch, dtype, direction)
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", [16, 64, 81, 243, 256, 361, 512, 2048, 4096, 8192, 16384, 23, 67, 997, 1009])
+def test_batch_256(gpu, n, dtype):
+    """Batch 256 of the reference's "_b" binaries (ctest/flagfft_test.h:509-526), both directions."""
+    for direction in (ff.FORWARD, ff.INVERSE):
+        _case(gpu, n, 256, dtype, direction)
+
+
 @pytest.mark.parametrize("scale", [2.0 ** -20, 2.0 ** 20])
 @pytest.mark.parametrize("n", [64, 997, 4096, 16384])
 def test_input_scales(gpu, n, scale):

@@ -118,3 +118,46 @@ def test_padded_in_place_r2c_c2r(gpu, n):
         sync(gpu)
     z = host(buf)[:, :n]
     check_accuracy(z, x.astype(np.float64) * n, ff.Type.C2R, n, row_len=n, factor=2.0, what="in-place C2R")
+
+
+def test_c2r_large_batch_multiple_seeds(gpu):
+    """ctest/test_exec_c2r_ct_b.cpp:76-117: N=16384, batch 256, seeds 1..10, Im of DC / Nyquist zeroed."""
+    n, batch = 16384, 256
+    h = n // 2 + 1
+    for seed in range(1, 11):
+        X = complex_input(ora.accuracy_seed(int(ff.Type.C2R), n, batch, seed), (batch, h), np.complex64)
+        X[:, 0] = X[:, 0].real
+        X[:, n // 2] = X[:, n // 2].real
+        y = _c2r(gpu, X, n, batch, False)
+        check_accuracy(y, ora.irfft(X.astype(np.complex128), n), ff.Type.C2R, n, row_len=n, what=f"seed={seed}")
+
+
+# batch 256 of the reference's "_b" binaries (ctest/flagfft_test.h:509-526)
+BATCH_SIZES = [16, 64, 81, 243, 256, 361, 512, 2048, 4096]
+
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["r2c-c2r", "d2z-z2d"])
+@pytest.mark.parametrize("n", BATCH_SIZES)
+def test_real_batch_256(gpu, n, dbl):
+    tf, ti = (ff.Type.D2Z, ff.Type.Z2D) if dbl else (ff.Type.R2C, ff.Type.C2R)
+    batch = 256
+    x = real_input(ora.accuracy_seed(int(tf), n, batch), (batch, n), np.float64 if dbl else np.float32)
+    X = _r2c(gpu, x, n, batch, dbl)
+    check_accuracy(X, ora.rfft(x.astype(np.float64)), tf, n, row_len=n // 2 + 1, what="R2C batch 256")
+    Y = complex_input(ora.accuracy_seed(int(ti), n, batch), (batch, n // 2 + 1), np.complex128 if dbl else np.complex64)
+    y = _c2r(gpu, Y, n, batch, dbl)
+    check_accuracy(y, ora.irfft(Y.astype(np.complex128), n), ti, n, row_len=n, what="C2R batch 256")
+
+
+@pytest.mark.parametrize("scale", [2.0 ** -20, 2.0 ** 20])
+@pytest.mark.parametrize("n", [64, 997, 4096, 6000])
+def test_real_input_scales(gpu, n, scale):
+    """Inputs scaled by 2^-20 and 2^20 (ctest/flagfft_test.h:457), both precisions."""
+    for dbl, tf, ti in ((False, ff.Type.R2C, ff.Type.C2R), (True, ff.Type.D2Z, ff.Type.Z2D)):
+        x = real_input(ora.accuracy_seed(int(tf), n, 4), (4, n), np.float64 if dbl else np.float32, scale=scale)
+        X = _r2c(gpu, x, n, 4, dbl)
+        check_accuracy(X, ora.rfft(x.astype(np.float64)), tf, n, row_len=n // 2 + 1, what=f"scale={scale}")
+        Y = complex_input(ora.accuracy_seed(int(ti), n, 4), (4, n // 2 + 1), np.complex128 if dbl else np.complex64,
+                          scale=scale)
+        y = _c2r(gpu, Y, n, 4, dbl)
+        check_accuracy(y, ora.irfft(Y.astype(np.complex128), n), ti, n, row_len=n, what=f"C2R scale={scale}")
