@@ -746,6 +746,9 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   stage_kernel_shape(sid, dbl(), cu, ept);
   const int tpu = (R + ept - 1) / ept;
   const int LS = odd_stride(lds_pad(R - 1) + 1);
+  // the instance's launch bounds and the LDS of one CU
+  if (cu * tpu > kMaxBlockThreads || static_cast<size_t>(cu) * LS * eb() > static_cast<size_t>(kLdsBytesMax))
+    return false;
   Step sst;
   sst.kind = Step::Stage;
   sst.dbl = dbl();
