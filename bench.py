@@ -18,7 +18,10 @@ value          algorithmic HBM GB/s of the whole step: sum over transforms of
 roofline       dominant kernel (by total device time): algorithmic bytes per
                launch / mean launch duration from HIP events recorded on the
                plan's stream around every launch of a second timed region of
-               the same K steps (the first, event-free region gives value)
+               the same K steps (the first, event-free region gives value);
+               traffic = counter-measured HBM bytes per launch (committed
+               rocprofv3 summary), traffic_gbps / traffic_frac = those bytes
+               over the same duration
 cpu_baseline   the oracle's fast FFT (oracle/oracle.c) on the host cores over
                a bounded sample of the same step
 
@@ -374,12 +377,19 @@ def roofline_of(launches):
     avg_ms = d["ms"] / d["calls"]
     bytes_per_launch = d["bytes"] / d["calls"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    return {
+    traffic = traffic_of(sym)
+    roof = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(sym),
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         "kernel": sym, "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": int(bytes_per_launch),
         "launches": d["calls"], "share_of_device_time": round(d["ms"] / sum(v["ms"] for v in by_sym.values()), 3),
-    }, {k: {"ms": round(v["ms"], 3), "calls": v["calls"], "avg_us": round(v["ms"] / max(v["calls"], 1) * 1e3, 2)}
+    }
+    if traffic:
+        # HBM bytes the counters see per launch over the same mean duration: how
+        # close the kernel's actual traffic comes to the HBM roof
+        roof["traffic_gbps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+        roof["traffic_frac"] = round(roof["traffic_gbps"] / HBM_PEAK_GBS, 4)
+    return roof, {k: {"ms": round(v["ms"], 3), "calls": v["calls"], "avg_us": round(v["ms"] / max(v["calls"], 1) * 1e3, 2)}
         for k, v in by_sym.items()}
 
 
