@@ -334,6 +334,20 @@ extern "C" flagfftResult flagfftxPlanMany(flagfftHandle *plan, int rank, int *n,
   flagfftResult r = flagfft::make_desc(rank, n, inembed, istride, idist, onembed, ostride, odist, type, batch, d);
   if (r == FLAGFFT_SUCCESS) return flagfft::build_plan(plan, d);
   if (r != FLAGFFT_NOT_SUPPORTED) return r;
+  if (rank == 2 && (d.real_fwd || d.real_inv)) {
+    // real 2D with padded rows: unit strides, batches n0 rows apart
+    const int64_t n0 = d.n[0], n1 = d.n[1], h1 = n1 / 2 + 1;
+    const int64_t iemb = d.inembed[1], oemb = d.onembed[1];
+    const int64_t imin = d.real_fwd ? n1 : h1, omin = d.real_fwd ? h1 : n1;
+    if (istride != 1 || ostride != 1) return FLAGFFT_NOT_SUPPORTED;
+    if (iemb < imin || oemb < omin || idist != n0 * iemb || odist != n0 * oemb) return FLAGFFT_NOT_SUPPORTED;
+    // pairs of reals are read / written as complex values: even real pitch
+    if (n1 % 2 == 0 && (d.real_fwd ? iemb : oemb) % 2 != 0) return FLAGFFT_NOT_SUPPORTED;
+    d.general = true;
+    d.strides.irow = iemb;
+    d.strides.orow = oemb;
+    return flagfft::build_plan(plan, d);
+  }
   if ((type != FLAGFFT_C2C && type != FLAGFFT_Z2Z) || rank > 2) return FLAGFFT_NOT_SUPPORTED;
   if (istride <= 0 || ostride <= 0 || idist <= 0 || odist <= 0) return FLAGFFT_INVALID_VALUE;
   const int64_t last = d.n[static_cast<size_t>(rank - 1)];

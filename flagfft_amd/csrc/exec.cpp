@@ -1518,7 +1518,34 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
   auto ex = std::make_unique<Executable>();
   Builder b(t.desc.prec, o, *ex, nullptr);
   const int64_t batch = t.desc.batch;
-  if (st != nullptr) {
+  if (st != nullptr && t.desc.rank == 2 && (t.desc.real_fwd || t.desc.real_inv)) {
+    // real 2D with padded rows (row pitch irow / orow in the side's own
+    // elements, batches n0 rows apart, unit element stride), e.g. the
+    // in-place layout of 2 (n1 / 2 + 1) reals per row
+    const int64_t n0 = t.desc.n0, h1 = t.desc.n1 / 2 + 1;
+    const int W = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
+    SeqSet r;
+    r.nbatch = batch * n0;
+    SeqSet c;
+    c.nbatch = batch;
+    c.ncol = h1;
+    c.col_fast = true;
+    if (t.desc.real_fwd) {
+      r.gi = Geometry{1, 0, st->irow};
+      r.go = Geometry{1, 0, h1};
+      b.emit_plan1d(t.rows, r, kBufIn, W, false);
+      c.gi = Geometry{h1, 1, n0 * h1};
+      c.go = Geometry{st->orow, 1, n0 * st->orow};
+      b.emit_plan1d(t.cols, c, W, kBufOut, false);
+    } else {
+      c.gi = Geometry{st->irow, 1, n0 * st->irow};
+      c.go = Geometry{h1, 1, n0 * h1};
+      b.emit_plan1d(t.cols, c, kBufIn, W, true);
+      r.gi = Geometry{1, 0, h1};
+      r.go = Geometry{1, 0, st->orow};
+      b.emit_plan1d(t.rows, r, W, kBufOut, true);
+    }
+  } else if (st != nullptr) {
     // general complex layouts: the run-time-radix and LDS kernels take any
     // element / row / batch stride; the specialised kernels take the
     // contiguous cases only

@@ -116,13 +116,17 @@ flagfftResult flagfftxPlan3d(flagfftHandle *plan, int nx, int ny, int nz, flagff
 /* flagfftPlanMany with the advanced layouts as well: for FLAGFFT_C2C /
  * FLAGFFT_Z2Z of rank 1 or 2, any element stride >= 1, distance >= 1 and
  * embed (embed[rank-1] >= n[rank-1]; element y of row x of batch entry b at
- * b * dist + (x * embed[1] + y) * stride), which the reference answers with
- * FLAGFFT_NOT_SUPPORTED (src/exec/c_api_internal.cpp:60-120; flagfftPlanMany
- * keeps that answer here).  Layouts flagfftPlanMany accepts behave exactly as
- * there; other real layouts and rank 3 stay FLAGFFT_NOT_SUPPORTED; a stride,
- * distance or embed out of range is FLAGFFT_INVALID_VALUE.  No counterpart in
- * the reference.  The output layout must not overlap the input unless both
- * are the same layout in the same buffer. */
+ * b * dist + (x * embed[1] + y) * stride); for the real types of rank 2,
+ * padded rows (unit strides, embed[1] >= the row length of that side,
+ * dist = n[0] * embed[1], an even real-side pitch when n[1] is even; the
+ * in-place layout of 2 (n[1]/2+1) reals per row is one case).  The reference
+ * answers all of these with FLAGFFT_NOT_SUPPORTED
+ * (src/exec/c_api_internal.cpp:60-120; flagfftPlanMany keeps that answer
+ * here).  Layouts flagfftPlanMany accepts behave exactly as there; other real
+ * layouts and rank 3 stay FLAGFFT_NOT_SUPPORTED; a complex stride, distance
+ * or embed out of range is FLAGFFT_INVALID_VALUE.  No counterpart in the
+ * reference.  The output layout must not overlap the input unless both are
+ * the same layout in the same buffer (or the real in-place layout). */
 flagfftResult flagfftxPlanMany(flagfftHandle *plan, int rank, int *n, int *inembed, int istride, int idist,
                                int *onembed, int ostride, int odist, flagfftType type, int batch);
 

@@ -122,11 +122,21 @@ def test_advanced_layout_extension_validation():
     assert ff.plan_many_ext(1, [256], None, 2, 0, None, 1, 256, T.C2C, 1)[0] == R.INVALID_VALUE
     assert ff.plan_many_ext(2, [64, 32], [64, 31], 1, 4096, None, 1, 2048, T.C2C, 1)[0] == R.INVALID_VALUE
     assert ff.plan_many_ext(1, [256], None, 2, 600, None, 1, 256, T.R2C, 1)[0] == R.NOT_SUPPORTED
+    # real 2D: padded rows with unit strides only, batches n0 rows apart, even real pitch
+    assert ff.plan_many_ext(2, [16, 32], [16, 34], 2, 16 * 34, [16, 17], 1, 16 * 17, T.R2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many_ext(2, [16, 32], [16, 34], 1, 16 * 34 + 1, [16, 17], 1, 16 * 17, T.R2C, 1)[0] == \
+        R.NOT_SUPPORTED
+    assert ff.plan_many_ext(2, [16, 32], [16, 35], 1, 16 * 35, [16, 17], 1, 16 * 17, T.R2C, 1)[0] == R.NOT_SUPPORTED
+    assert ff.plan_many(2, [16, 32], [16, 34], 1, 16 * 34, [16, 17], 1, 16 * 17, T.R2C, 1)[0] == R.NOT_SUPPORTED
     assert ff.plan_many_ext(3, [8, 8, 8], None, 2, 2048, None, 1, 512, T.C2C, 1)[0] == R.NOT_SUPPORTED
     assert ff.lib().flagfftxPlanMany(None, 1, None, None, 1, 1, None, 1, 1, int(T.C2C), 1) == R.INVALID_VALUE
     if not torch.cuda.is_available():
         assert ff.plan_many_ext(*strided)[0] == R.INVALID_DEVICE
         assert ff.plan_many_ext(1, [256], None, 3, 800, None, 2, 600, T.Z2Z, 4)[0] == R.INVALID_DEVICE
+        assert ff.plan_many_ext(2, [16, 32], [16, 34], 1, 16 * 34, [16, 17], 1, 16 * 17, T.R2C, 2)[0] == \
+            R.INVALID_DEVICE
+        assert ff.plan_many_ext(2, [16, 32], [16, 17], 1, 16 * 17, [16, 34], 1, 16 * 34, T.C2R, 2)[0] == \
+            R.INVALID_DEVICE
 
 
 def test_compute_mode_validation():

@@ -1,6 +1,6 @@
 """Advanced layouts through flagfftxPlanMany (GPU): element strides, embeds and
-distances for C2C / Z2Z of rank 1 and 2, against the oracle on the gathered
-logical data.  Every position of the output buffer outside the output layout
+distances for C2C / Z2Z of rank 1 and 2, and padded rows for real rank 2,
+against the oracle on the gathered logical data.  Every position of the output buffer outside the output layout
 must keep its sentinel value (no write outside the layout).
 Bounds: FP32 2e-6 log2 N, FP64 1e-14 log2 N relative L2 per transform.
 """
@@ -10,7 +10,7 @@ import pytest
 from flagfft_amd import flagfft as ff
 from oracle import oracle as ora
 
-from .helpers import check_accuracy, complex_input, dev_tensor, host, stream_of, sync
+from .helpers import check_accuracy, complex_input, dev_tensor, host, real_input, stream_of, sync
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +98,62 @@ def test_reference_layouts_behave_as_plan_many(gpu):
                  (2, [64, 32], None, 1, 2048, None, 1, 2048, ff.Type.Z2Z, 2)):
         with ff.Plan.many(*args) as a, ff.Plan.many_ext(*args) as b:
             assert a.description() == b.description()
+
+
+REAL_2D = [(64, 48), (23, 16), (128, 100), (16, 997), (32, 15), (4096, 64)]
+
+
+@pytest.mark.parametrize("dbl", [False, True], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("shape", REAL_2D, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_real_2d_padded_rows(gpu, shape, dbl):
+    """Real 2D with padded rows through flagfftxPlanMany: the in-place layout of
+    2 (n1/2+1) reals per row (R2C then C2R in one buffer) and out-of-place rows
+    with extra pitch on both sides; batch 2, n0 rows per batch entry."""
+    import torch
+
+    n0, n1 = shape
+    h = n1 // 2 + 1
+    n = n0 * n1
+    batch = 2
+    tf, ti = (ff.Type.D2Z, ff.Type.Z2D) if dbl else (ff.Type.R2C, ff.Type.C2R)
+    rdt, cdt = (np.float64, np.complex128) if dbl else (np.float32, np.complex64)
+    tdt = torch.float64 if dbl else torch.float32
+    x = real_input(n + 17, (batch, n0, n1), rdt, kind="normal")
+    ref = ora.rfft2(x.astype(np.float64))
+    # in place: rows of 2h reals / h complex
+    buf = torch.zeros(batch * n0 * 2 * h, dtype=tdt, device=gpu)
+    buf.view(batch, n0, 2 * h)[:, :, :n1] = torch.from_numpy(x).to(gpu)
+    pad_r, pad_c = 2 * h, h
+    with ff.Plan.many_ext(2, [n0, n1], [n0, pad_r], 1, n0 * pad_r, [n0, pad_c], 1, n0 * pad_c, tf, batch) as pf, \
+            ff.Plan.many_ext(2, [n0, n1], [n0, pad_c], 1, n0 * pad_c, [n0, pad_r], 1, n0 * pad_r, ti, batch) as pi:
+        assert "layout:" in pf.description()
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        (pf.d2z if dbl else pf.r2c)(buf.data_ptr(), buf.data_ptr())
+        sync(gpu)
+        X = host(buf).view(cdt).reshape(batch, n0, h)
+        check_accuracy(X, ref, tf, n, row_len=n0 * h, what="in-place padded R2C")
+        (pi.z2d if dbl else pi.c2r)(buf.data_ptr(), buf.data_ptr())
+        sync(gpu)
+        z = host(buf).reshape(batch, n0, 2 * h)[:, :, :n1]
+        check_accuracy(z, x.astype(np.float64) * n, ti, n, row_len=n, factor=2.0, what="in-place padded roundtrip")
+    # out of place: input rows n1 + 6 reals, output rows h + 3 complex, and back
+    pin, pout = n1 + 6, h + 3
+    xin = np.full((batch, n0, pin), 7.0, rdt)
+    xin[:, :, :n1] = x
+    xd = dev_tensor(xin, gpu)
+    Yd = dev_tensor(np.full((batch, n0, pout), SENTINEL, cdt), gpu)
+    zd = dev_tensor(np.full((batch, n0, pin), -3.0, rdt), gpu)
+    with ff.Plan.many_ext(2, [n0, n1], [n0, pin], 1, n0 * pin, [n0, pout], 1, n0 * pout, tf, batch) as pf, \
+            ff.Plan.many_ext(2, [n0, n1], [n0, pout], 1, n0 * pout, [n0, pin], 1, n0 * pin, ti, batch) as pi:
+        pf.set_stream(stream_of(gpu))
+        pi.set_stream(stream_of(gpu))
+        (pf.d2z if dbl else pf.r2c)(xd.data_ptr(), Yd.data_ptr())
+        (pi.z2d if dbl else pi.c2r)(Yd.data_ptr(), zd.data_ptr())
+        sync(gpu)
+    Y = host(Yd)
+    check_accuracy(Y[:, :, :h], ref, tf, n, row_len=n0 * h, what="padded R2C")
+    assert np.all(Y[:, :, h:] == SENTINEL), "R2C wrote into the row padding"
+    z = host(zd)
+    check_accuracy(z[:, :, :n1], x.astype(np.float64) * n, ti, n, row_len=n, factor=2.0, what="padded roundtrip")
+    assert np.all(z[:, :, n1:] == -3.0), "C2R wrote into the row padding"
