@@ -96,40 +96,64 @@ __device__ __forceinline__ int out_row(int i) { return 8 * (i >> 3) + 2 * (i & 3
 __device__ __forceinline__ V2<float> w1024(const GlbC<float> *tw, int e) { return ldg(tw + (e & 1023)); }
 
 template <bool F16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma1024(const MfmaArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_mfma1024(const MfmaArgs a) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 31;
   const int h = lane >> 5;
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
   const GlbC<float> *tw = gc<float>(a.tw);
+  using V8 = typename HalfT<F16>::V8;
 
-  // per-lane constant operands (identical for every transform this wave handles)
-  Split8<F16> fb_r[2], fb_i[2];  // step 1 B operand: F[16q + 2v + h][c]
-  Split8<F16> fa_r[2], fa_i[2];  // step 2 A operand: F[pi(c)][16q + 8(v/4) + 4h + v%4]
-  V2<float> twid[16];            // w_1024^(n2 c), n2 = 8(r/4) + 4h + r%4
-  const int pc = out_row(c);
+  // per-lane constant operands (identical for every wave) in LDS, 24 x 16 B
+  // per lane, read back at each use: registers for a third wave per SIMD
+  __shared__ uint4 kc[24][64];
+  if (threadIdx.x < 64) {
+    const int pc = out_row(c);
+    Split8<F16> fb_r[2], fb_i[2], fa_r[2], fa_i[2];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 2; ++q) {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const V2<float> f1 = w1024(tw, 32 * (16 * q + 2 * v + h) * c);
-      const V2<float> f2 = w1024(tw, 32 * pc * (16 * q + 8 * (v >> 2) + 4 * h + (v & 3)));
-      split_into<F16>(fb_r[q], v, f1.x);
-      split_into<F16>(fb_i[q], v, f1.y);
-      split_into<F16>(fa_r[q], v, f2.x);
-      split_into<F16>(fa_i[q], v, f2.y);
+      for (int v = 0; v < 8; ++v) {
+        const V2<float> f1 = w1024(tw, 32 * (16 * q + 2 * v + h) * c);
+        const V2<float> f2 = w1024(tw, 32 * pc * (16 * q + 8 * (v >> 2) + 4 * h + (v & 3)));
+        split_into<F16>(fb_r[q], v, f1.x);
+        split_into<F16>(fb_i[q], v, f1.y);
+        split_into<F16>(fa_r[q], v, f2.x);
+        split_into<F16>(fa_i[q], v, f2.y);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      kc[0 + 2 * q][lane] = __builtin_bit_cast(uint4, fb_r[q].hi);
+      kc[1 + 2 * q][lane] = __builtin_bit_cast(uint4, fb_r[q].lo);
+      kc[4 + 2 * q][lane] = __builtin_bit_cast(uint4, fb_i[q].hi);
+      kc[5 + 2 * q][lane] = __builtin_bit_cast(uint4, fb_i[q].lo);
+      kc[8 + 2 * q][lane] = __builtin_bit_cast(uint4, fa_r[q].hi);
+      kc[9 + 2 * q][lane] = __builtin_bit_cast(uint4, fa_r[q].lo);
+      kc[12 + 2 * q][lane] = __builtin_bit_cast(uint4, fa_i[q].hi);
+      kc[13 + 2 * q][lane] = __builtin_bit_cast(uint4, fa_i[q].lo);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+      const V2<float> t0 = w1024(tw, (8 * (r >> 2) + 4 * h + (r & 3)) * c);
+      const V2<float> t1 = w1024(tw, (8 * ((r + 1) >> 2) + 4 * h + ((r + 1) & 3)) * c);
+      kc[16 + r / 2][lane] = uint4{__float_as_uint(t0.x), __float_as_uint(t0.y), __float_as_uint(t1.x),
+                                   __float_as_uint(t1.y)};
     }
   }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) twid[r] = w1024(tw, (8 * (r >> 2) + 4 * h + (r & 3)) * c);
+  __syncthreads();
+  auto ks = [&](int i) __attribute__((always_inline)) {
+    return Split8<F16>{__builtin_bit_cast(V8, kc[i][lane]), __builtin_bit_cast(V8, kc[i + 1][lane])};
+  };
 
   const float cin = a.conj_in ? -1.f : 1.f;
   const float cout = a.conj_out ? -1.f : 1.f;
   const float user_scale = static_cast<float>(a.scale);
 
   // software pipeline: the next transform's loads are in flight while this
-  // one runs through the matrix cores (two waves per SIMD fit the registers)
+  // one runs through the matrix cores (three waves per SIMD fit the registers
+  // with the constant operands in LDS)
   auto load = [&](int64_t sq, V2<float> (&x)[2][8]) __attribute__((always_inline)) {
     const GlbC<float> *src = gc<float>(a.in) + sq * a.idist + c;
 #pragma unroll
@@ -147,6 +171,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
       for (int v = 0; v < 8; ++v) x[q][v] = xn[q][v];
     if (seq + nwaves < a.nseq) load(seq + nwaves, xn);
+    asm volatile("" ::: "memory");  // the constants are read from LDS in every iteration
 
     // exact power-of-two range scaling (FP16 only)
     float pre = 1.f, post = 1.f;
@@ -179,17 +204,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     f32x16 br = {}, bi = {};
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      br = mma3<F16>(ar[q], fb_r[q], br);
-      br = mma3<F16>(neg<F16>(ai[q]), fb_i[q], br);
-      bi = mma3<F16>(ar[q], fb_i[q], bi);
-      bi = mma3<F16>(ai[q], fb_r[q], bi);
+      const Split8<F16> fbr = ks(0 + 2 * q), fbi = ks(4 + 2 * q);
+      br = mma3<F16>(ar[q], fbr, br);
+      br = mma3<F16>(neg<F16>(ai[q]), fbi, br);
+      bi = mma3<F16>(ar[q], fbi, bi);
+      bi = mma3<F16>(ai[q], fbr, bi);
     }
 
     // twiddle in FP32, then split into step 2's B operand (register 8q + v)
     Split8<F16> sr[2], si[2];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const V2<float> t = twid[r];
+      const uint4 tt = kc[16 + r / 2][lane];
+      const V2<float> t = (r & 1) ? V2<float>{__uint_as_float(tt.z), __uint_as_float(tt.w)}
+                                  : V2<float>{__uint_as_float(tt.x), __uint_as_float(tt.y)};
       const float re = br[r] * t.x - bi[r] * t.y;
       const float im = br[r] * t.y + bi[r] * t.x;
       split_into<F16>(sr[r >> 3], r & 7, re);
@@ -200,10 +228,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     f32x16 cr = {}, ci = {};
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      cr = mma3<F16>(fa_r[q], sr[q], cr);
-      cr = mma3<F16>(fa_i[q], neg<F16>(si[q]), cr);
-      ci = mma3<F16>(fa_r[q], si[q], ci);
-      ci = mma3<F16>(fa_i[q], sr[q], ci);
+      const Split8<F16> far = ks(8 + 2 * q), fai = ks(12 + 2 * q);
+      cr = mma3<F16>(far, sr[q], cr);
+      cr = mma3<F16>(fai, neg<F16>(si[q]), cr);
+      ci = mma3<F16>(far, si[q], ci);
+      ci = mma3<F16>(fai, sr[q], ci);
     }
 
     const float so = post * user_scale;
