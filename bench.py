@@ -229,9 +229,9 @@ def lpt(times, nq):
     return slot
 
 
-def pick_streams(cases, times, torch, dev, nq_max, steps=40):
+def pick_streams(cases, times, torch, dev, nq_max, steps=100):
     """Number of streams for the families: every count from 1 to nq_max with the
-    longest-first assignment, timed over `steps` steps (best of two regions);
+    longest-first assignment, timed over `steps` steps (best of three regions);
     the fastest count wins.  Returns (streams, slot, {count: us per step})."""
     table = {}
     best = None
@@ -241,11 +241,11 @@ def pick_streams(cases, times, torch, dev, nq_max, steps=40):
         for i, c in enumerate(cases):
             c.stream = streams[slot[i]]
             c.bind(c.stream.cuda_stream)
-        for s in range(3):
+        for s in range(10):
             for c in cases:
                 c.run(s)
         t = float("inf")
-        for _ in range(2):
+        for _ in range(3):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for s in range(steps):
@@ -283,7 +283,7 @@ def measure(cases, steps, warmup, torch, ff, stream, case_streams, slot):
         return time.perf_counter() - t0
 
     bind(True)
-    for s in range(warmup):
+    for s in range(max(warmup, 20)):
         for c in cases:
             c.run(s)
     t_multi = region(warmup)
