@@ -160,3 +160,22 @@ def test_c2c_3d_in_place_and_graph(gpu):
             sync(gpu)
             assert np.array_equal(host(gd), host(yd)), f"graph replay {rep}"
     check_accuracy(host(yd), ref, ff.Type.C2C, n, row_len=n, factor=2.0, what="3D")
+
+
+def test_3d_exec_contract(gpu):
+    """Exec on a rank-3 plan follows the reference's Exec contract (src/exec/c_api_exec.cpp:9-21):
+    wrong type -> INVALID_TYPE, null pointer -> INVALID_VALUE, bad direction -> INVALID_VALUE,
+    destroyed handle -> INVALID_PLAN."""
+    import torch
+
+    R = ff.Result
+    x = torch.zeros(2 * 4 * 6 * 8, dtype=torch.complex64, device=gpu)
+    p = ff.Plan.three_d(4, 6, 8, ff.Type.C2C, 2)
+    h = p.handle
+    assert ff.exec_c2c(h, x.data_ptr(), x.data_ptr(), 7) == R.INVALID_VALUE
+    assert ff.exec_c2c(h, 0, x.data_ptr(), ff.FORWARD) == R.INVALID_VALUE
+    assert ff.exec_z2z(h, x.data_ptr(), x.data_ptr(), ff.FORWARD) == R.INVALID_TYPE
+    assert ff.exec_c2c(h, x.data_ptr(), x.data_ptr(), ff.FORWARD) == R.SUCCESS
+    sync(gpu)
+    p.destroy()
+    assert ff.exec_c2c(h, x.data_ptr(), x.data_ptr(), ff.FORWARD) == R.INVALID_PLAN
