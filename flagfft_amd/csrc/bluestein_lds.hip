@@ -367,6 +367,7 @@ __device__ __forceinline__ void bluestein_split_body(const BluesteinLdsArgs &a) 
   const int n = io.n;
   const GlbC<T> *tws = gc<T>(a.tw);
   GlbC<T> *park = gc<T>(a.park) + static_cast<int64_t>(slot) * ML;
+  const GlbC<T> *cws = gc<T>(a.chirp_split) + static_cast<int64_t>(h) * n;
   conv_lds<T, TPU, false, P, RS::kR[0], RS::kR[0], RS>(
       lds_base<T>(), threadIdx.x, tws, gc<T>(a.bhat) + h * ML,
       [&](int e) __attribute__((always_inline)) {
@@ -374,7 +375,16 @@ __device__ __forceinline__ void bluestein_split_body(const BluesteinLdsArgs &a) 
 #pragma unroll
         for (int j = 0; j < P / 2; ++j) {
           const int e2 = e + j * ML;
-          if (e2 < n) v = cadd(v, cmul(io.in(e2), ldt(tws, static_cast<uint32_t>((h * e2) % M))));
+          if (e2 < n) {
+            if constexpr (kFast) {
+              // x[e2] chirp[e2] w_M^(h e2) from one FP64-generated table
+              V2<T> xv = ldt(gc<T>(a.in) + io.ibase, static_cast<uint32_t>(e2));
+              if (a.conj_in) xv.y = -xv.y;
+              v = cadd(v, cmul(xv, ldt(cws, static_cast<uint32_t>(e2))));
+            } else {
+              v = cadd(v, cmul(io.in(e2), ldt(tws, static_cast<uint32_t>((h * e2) % M))));
+            }
+          }
         }
         return v;
       },

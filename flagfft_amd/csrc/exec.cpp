@@ -1004,6 +1004,17 @@ void Builder::emit_bluestein_split(const Core &c, const SeqSet &s, int src, int 
   a.tw = tw;
   a.chirp = ch;
   a.bhat = bhat_bluestein_digit_reversed(n, M, digits);
+  {
+    const std::string key = "chirpsplit:" + std::to_string(eb()) + ":" + std::to_string(n) + ":" +
+                            std::to_string(M) + ":" + std::to_string(split);
+    auto it = ex_.tables.find(key);
+    if (it == ex_.tables.end()) {
+      auto buf = std::make_shared<DevBuf>(static_cast<size_t>(n * split) * eb());
+      hip_check(gen_chirp_split(buf->ptr, dbl(), n, M, split, st_), "gen_chirp_split");
+      it = ex_.tables.emplace(key, buf).first;
+    }
+    a.chirp_split = it->second->ptr;
+  }
   a.src_mode = m.src_mode & 3;
   a.dst_mode = m.dst_mode & 3;
   a.conj_in = m.conj_in;

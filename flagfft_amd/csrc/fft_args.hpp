@@ -184,6 +184,7 @@ struct BluesteinLdsArgs {
   const void *chirp;     // n entries
   const void *bhat;      // FFT_M(b) / M
   void *park;            // two-halves kernels: n elements per sequence
+  const void *chirp_split;  // split kernels: chirp[k] w_M^(h k), P blocks of n
   int32_t src_mode, dst_mode;
   int32_t conj_in, conj_out;
   double scale;

@@ -762,6 +762,25 @@ __global__ void k_gen_chirp(V2<T> *out, int64_t n) {
   }
 }
 
+// chirp[k] w_m^(h k) for h < p, k < n (FP64 phase, one rounding): the
+// first-stage factors of a convolution split over p workgroups.
+template <typename T>
+__global__ void k_gen_chirp_split(V2<T> *out, int64_t n, int64_t m, int p) {
+  const int64_t total = n * p;
+  for (int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; j < total;
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t h = j / n, k = j - h * n;
+    const uint64_t r = (static_cast<uint64_t>(k) * static_cast<uint64_t>(k)) % static_cast<uint64_t>(2 * n);
+    const int64_t t = (h * k) % m;
+    // exp(-i pi r / n) exp(-2 i pi t / m), phase reduced to [-2, 0] half-turns
+    double ph = -static_cast<double>(r) / static_cast<double>(n) - 2.0 * static_cast<double>(t) / static_cast<double>(m);
+    if (ph < -2.0) ph += 2.0;
+    double s, c;
+    sincospi(ph, &s, &c);
+    out[j] = V2<T>{static_cast<T>(c), static_cast<T>(s)};
+  }
+}
+
 __global__ void k_gen_bluestein_b(V2<double> *out, int64_t n, int64_t m) {
   for (int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; j < m;
        j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -855,6 +874,15 @@ hipError_t gen_roots(void *out, bool dbl, int64_t count, int64_t n, int64_t mult
   dim3 g = grid_for(count, 256);
   if (dbl) hipLaunchKernelGGL(dev::k_gen_roots<double>, g, dim3(256), 0, st, static_cast<dev::V2<double> *>(out), count, n, mult);
   else hipLaunchKernelGGL(dev::k_gen_roots<float>, g, dim3(256), 0, st, static_cast<dev::V2<float> *>(out), count, n, mult);
+  return hipGetLastError();
+}
+
+hipError_t gen_chirp_split(void *out, bool dbl, int64_t n, int64_t m, int p, hipStream_t st) {
+  const dim3 g = grid_for(n * p, 256);
+  if (dbl)
+    hipLaunchKernelGGL(dev::k_gen_chirp_split<double>, g, dim3(256), 0, st, static_cast<dev::V2<double> *>(out), n, m, p);
+  else
+    hipLaunchKernelGGL(dev::k_gen_chirp_split<float>, g, dim3(256), 0, st, static_cast<dev::V2<float> *>(out), n, m, p);
   return hipGetLastError();
 }
 

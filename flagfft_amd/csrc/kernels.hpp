@@ -93,6 +93,8 @@ hipError_t allow_lds(const void *f, size_t bytes);
 hipError_t gen_roots(void *out, bool dbl, int64_t count, int64_t n, int64_t mult, hipStream_t st);
 // out[k] = exp(-i pi (k^2 mod 2n) / n), k < n.
 hipError_t gen_chirp(void *out, bool dbl, int64_t n, hipStream_t st);
+// chirp[k] w_m^(h k), h < p, k < n (p blocks of n)
+hipError_t gen_chirp_split(void *out, bool dbl, int64_t n, int64_t m, int p, hipStream_t st);
 // FP64 Bluestein kernel b[k] = exp(+i pi (k^2 mod 2n)/n) for k < n, mirrored to M-k, zero elsewhere.
 hipError_t gen_bluestein_b(double2 *out, int64_t n, int64_t m, hipStream_t st);
 // FP64 Rader kernel b[q] = w_p^{idx[q]}, q < p-1.
