@@ -9,9 +9,10 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <new>
-#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -54,7 +55,10 @@ struct PlanImpl {
 };
 
 std::mutex g_handles_mu;
-std::set<flagfftHandle> g_handles;
+// live handles; a call holds its plan's reference until it returns, so a
+// Destroy that overlaps another call on the same handle frees the plan after
+// that call instead of under it
+std::map<flagfftHandle, std::shared_ptr<PlanImpl>> g_handles;
 thread_local PlanOptions t_options;
 thread_local bool t_explicit = false;  // flagfftxSetTuning overrides the tuned-plan database
 
@@ -188,11 +192,11 @@ PlanDesc plan_desc(const Desc &d) {
   return p;
 }
 
-PlanImpl *checked(flagfftHandle h) {
+std::shared_ptr<PlanImpl> checked(flagfftHandle h) {
   if (h == nullptr) return nullptr;
   std::lock_guard<std::mutex> lk(g_handles_mu);
-  if (g_handles.find(h) == g_handles.end()) return nullptr;
-  return static_cast<PlanImpl *>(h->impl);
+  auto it = g_handles.find(h);
+  return it == g_handles.end() ? nullptr : it->second;
 }
 
 flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
@@ -239,12 +243,13 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
     if (impl->fwd_ip) o << "\n-- Forward execution, in place --\n" << impl->fwd_ip->describe();
     if (impl->inv_ip) o << "\n-- Inverse execution, in place --\n" << impl->inv_ip->describe();
     impl->description = o.str();
+    std::shared_ptr<PlanImpl> shared(std::move(impl));
     auto handle = std::make_unique<flagfftPlan_t>();
-    handle->impl = impl.release();
+    handle->impl = shared.get();
     flagfftHandle h = handle.release();
     {
       std::lock_guard<std::mutex> lk(g_handles_mu);
-      g_handles.insert(h);
+      g_handles.emplace(h, std::move(shared));
     }
     *out = h;
     return FLAGFFT_SUCCESS;
@@ -261,7 +266,7 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
 
 flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagfftType expect, int direction,
                           bool has_direction) {
-  PlanImpl *p = checked(handle);
+  std::shared_ptr<PlanImpl> p = checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   if (in == nullptr || out == nullptr) return FLAGFFT_INVALID_VALUE;
   if (p->desc.type != expect) return FLAGFFT_INVALID_TYPE;
@@ -428,7 +433,7 @@ extern "C" flagfftResult flagfftExecZ2D(flagfftHandle plan, flagfftDoubleComplex
 }
 
 extern "C" flagfftResult flagfftSetStream(flagfftHandle handle, flagfftStream_t stream) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   std::lock_guard<std::mutex> lk(p->mu);
   p->stream = static_cast<hipStream_t>(stream);
@@ -437,24 +442,24 @@ extern "C" flagfftResult flagfftSetStream(flagfftHandle handle, flagfftStream_t 
 
 extern "C" flagfftResult flagfftDestroy(flagfftHandle handle) {
   if (handle == nullptr) return FLAGFFT_INVALID_PLAN;
+  std::shared_ptr<flagfft::PlanImpl> p;
   {
     std::lock_guard<std::mutex> lk(flagfft::g_handles_mu);
     auto it = flagfft::g_handles.find(handle);
     if (it == flagfft::g_handles.end()) return FLAGFFT_INVALID_PLAN;
+    p = std::move(it->second);
     flagfft::g_handles.erase(it);
-  }
-  auto *p = static_cast<flagfft::PlanImpl *>(handle->impl);
-  if (p != nullptr) {
-    { std::lock_guard<std::mutex> lk(p->mu); }
-    delete p;
   }
   handle->impl = nullptr;
   delete handle;
+  // the plan (workspaces, tables, graphs) goes with the last reference: here,
+  // or at the end of a call that looked the handle up before the erase
+  p.reset();
   return FLAGFFT_SUCCESS;
 }
 
 extern "C" const char *flagfftGetPlanDescription(flagfftHandle handle) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return nullptr;
   std::lock_guard<std::mutex> lk(p->mu);
   return p->description.c_str();
@@ -499,7 +504,7 @@ extern "C" flagfftResult flagfftxPlanDescribe(int rank, const int *n, flagfftTyp
 }
 
 extern "C" flagfftResult flagfftxPlanStats(flagfftHandle handle, int direction, int *launches, long long *pass_bytes) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   if (launches == nullptr || pass_bytes == nullptr) return FLAGFFT_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(p->mu);
@@ -526,7 +531,7 @@ Executable *exec_for(PlanImpl *p, int direction) {
 }  // namespace flagfft
 
 extern "C" flagfftResult flagfftxSetProfiling(flagfftHandle handle, int enable) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   std::lock_guard<std::mutex> lk(p->mu);
   if (p->fwd) p->fwd->set_profiling(enable != 0);
@@ -536,11 +541,11 @@ extern "C" flagfftResult flagfftxSetProfiling(flagfftHandle handle, int enable) 
 
 extern "C" flagfftResult flagfftxGetProfile(flagfftHandle handle, int direction, int capacity, double *ms_total,
                                            long long *calls, long long *bytes, int *nsteps) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   if (nsteps == nullptr || capacity < 0) return FLAGFFT_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(p->mu);
-  flagfft::Executable *ex = flagfft::exec_for(p, direction);
+  flagfft::Executable *ex = flagfft::exec_for(p.get(), direction);
   if (ex == nullptr) return FLAGFFT_INVALID_VALUE;
   if (ex->harvest_profile() != hipSuccess) return FLAGFFT_EXEC_FAILED;
   *nsteps = static_cast<int>(ex->steps.size());
@@ -553,16 +558,16 @@ extern "C" flagfftResult flagfftxGetProfile(flagfftHandle handle, int direction,
 }
 
 extern "C" const char *flagfftxStepLabel(flagfftHandle handle, int direction, int step) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return nullptr;
   std::lock_guard<std::mutex> lk(p->mu);
-  flagfft::Executable *ex = flagfft::exec_for(p, direction);
+  flagfft::Executable *ex = flagfft::exec_for(p.get(), direction);
   if (ex == nullptr || step < 0 || step >= static_cast<int>(ex->steps.size())) return nullptr;
   return ex->steps[static_cast<size_t>(step)].label.c_str();
 }
 
 extern "C" flagfftResult flagfftxSetGraphMode(flagfftHandle handle, int enable) {
-  flagfft::PlanImpl *p = flagfft::checked(handle);
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;
   std::lock_guard<std::mutex> lk(p->mu);
   p->graph_mode = enable != 0;
