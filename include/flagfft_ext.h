@@ -179,7 +179,9 @@ flagfftResult flagfftxServerWait(flagfftxServer server, long long ticket, void *
 flagfftResult flagfftxServerRelease(flagfftxServer server, long long ticket);
 flagfftResult flagfftxServerGetStats(flagfftxServer server, flagfftxServerStats *stats);
 flagfftResult flagfftxServerResetStats(flagfftxServer server);
-/* Waits for every submitted request, then frees the server. */
+/* Waits for every submitted request, then frees the server.  Unlike plan
+ * handles, a server handle is not looked up in a table: it must not be used
+ * after Destroy, nor destroyed while another thread is inside a call on it. */
 flagfftResult flagfftxServerDestroy(flagfftxServer server);
 
 #ifdef __cplusplus
