@@ -41,3 +41,68 @@ def test_past_2_31_elements(gpu, n, batch):
     finally:
         del x, y
         torch.cuda.empty_cache()
+
+
+def _rows(n, batch, elems_per_row):
+    """Sampled rows: first, last, and either side of the 2^29- and 2^31-element marks."""
+    rows = {0, batch - 1}
+    for mark in (1 << 29, 1 << 31):
+        r = mark // elems_per_row
+        rows |= {r - 1, r} if r < batch else set()
+    return sorted(rows)
+
+
+def test_bluestein_past_2_31_elements(gpu):
+    """N = 10007 (Bluestein, one workgroup's LDS per sequence), batch 214601."""
+    import torch
+
+    n, batch = 10007, 214601
+    assert n * batch > 1 << 31
+    gen = torch.Generator(device=gpu)
+    gen.manual_seed(n)
+    x = torch.randn(n * batch, dtype=torch.complex64, device=gpu, generator=gen)
+    y = torch.empty_like(x)
+    try:
+        with ff.Plan.one_d(n, ff.Type.C2C, batch) as p:
+            assert "Bluestein" in p.description()
+            p.set_stream(stream_of(gpu))
+            p.c2c(x.data_ptr(), y.data_ptr(), ff.FORWARD)
+            sync(gpu)
+        for r in _rows(n, batch, n):
+            xr = host(x[r * n:(r + 1) * n]).astype(np.complex128)
+            check_accuracy(host(y[r * n:(r + 1) * n])[None], ora.fft(xr[None]), ff.Type.C2C, n, row_len=n,
+                           what=f"N={n} sequence {r}")
+    finally:
+        del x, y
+        torch.cuda.empty_cache()
+
+
+def test_real_past_2_31_elements(gpu):
+    """R2C then C2R at N = 6000, batch 357914: 2^31 + 352 reals on the real side,
+    rows sampled at the marks of both the real and the half-complex buffers."""
+    import torch
+
+    n, batch = 6000, 357914
+    h = n // 2 + 1
+    assert n * batch > 1 << 31
+    gen = torch.Generator(device=gpu)
+    gen.manual_seed(n + 1)
+    x = torch.rand(n * batch, dtype=torch.float32, device=gpu, generator=gen) * 2 - 1
+    X = torch.empty(h * batch, dtype=torch.complex64, device=gpu)
+    z = torch.empty_like(x)
+    try:
+        with ff.Plan.one_d(n, ff.Type.R2C, batch) as pf, ff.Plan.one_d(n, ff.Type.C2R, batch) as pi:
+            pf.set_stream(stream_of(gpu))
+            pi.set_stream(stream_of(gpu))
+            pf.r2c(x.data_ptr(), X.data_ptr())
+            pi.c2r(X.data_ptr(), z.data_ptr())
+            sync(gpu)
+        for r in sorted(set(_rows(n, batch, n)) | set(_rows(n, batch, h))):
+            xr = host(x[r * n:(r + 1) * n]).astype(np.float64)
+            check_accuracy(host(X[r * h:(r + 1) * h])[None], ora.rfft(xr[None]), ff.Type.R2C, n, row_len=h,
+                           what=f"R2C sequence {r}")
+            check_accuracy(host(z[r * n:(r + 1) * n])[None], xr[None] * n, ff.Type.C2R, n, row_len=n, factor=2.0,
+                           what=f"roundtrip sequence {r}")
+    finally:
+        del x, X, z
+        torch.cuda.empty_cache()
