@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -717,6 +718,9 @@ extern "C" flagfftResult flagfftxServerSubmit(flagfftxServer server, flagfftType
   if (server == nullptr) return FLAGFFT_INVALID_PLAN;
   if (idata == nullptr || ticket == nullptr || batch <= 0) return FLAGFFT_INVALID_VALUE;
   if (nx <= 0) return FLAGFFT_INVALID_SIZE;
+  // a request's plan is its prepared plan re-batched: unit and workgroup
+  // counts are at most its element count and are held in 32 bits
+  if (static_cast<int64_t>(nx) * batch > INT32_MAX) return FLAGFFT_INVALID_SIZE;
   int64_t t = 0;
   flagfftResult r = server->impl->submit(static_cast<int>(type), nx, batch, direction, idata, t);
   *ticket = t;

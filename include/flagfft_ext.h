@@ -169,6 +169,7 @@ typedef struct flagfftxServerStats_t {
 flagfftResult flagfftxServerCreate(flagfftxServer *server, size_t hbm_budget_bytes, int max_inflight, int mode);
 /* Build (and keep) the plans for one transform kind ahead of traffic. */
 flagfftResult flagfftxServerPrepare(flagfftxServer server, flagfftType type, int nx, int direction);
+/* nx * batch above 2^31 - 1 -> FLAGFFT_INVALID_SIZE (split such a request). */
 flagfftResult flagfftxServerSubmit(flagfftxServer server, flagfftType type, int nx, int batch, int direction,
                                    const void *idata, long long *ticket);
 /* done = 1 when complete; then *odata is the output in the arena. */

@@ -142,3 +142,13 @@ def test_request_larger_than_the_budget_is_refused(gpu):
         with pytest.raises(ff.FlagFFTError) as e:
             srv.submit(T.C2C, 4096, 4, x.data_ptr(), ff.FORWARD)
         assert e.value.code == ff.Result.ALLOC_FAILED
+
+
+def test_request_past_2_31_elements_is_refused(gpu):
+    """nx * batch above 2^31 - 1 is refused before anything is allocated or launched."""
+    x = dev_tensor(np.zeros((1, 64), np.complex64), gpu)
+    with ff.Server(1 << 20, max_inflight=1) as srv:
+        with pytest.raises(ff.FlagFFTError) as e:
+            srv.submit(T.C2C, 4096, (1 << 19) + 1, x.data_ptr(), ff.FORWARD)
+        assert e.value.code == ff.Result.INVALID_SIZE
+        assert srv.stats()["completed"] == 0
