@@ -3,8 +3,8 @@ batch 2^19 + 3) and the two-pass Stockham chain (N = 2^24, batch 129, with
 its workspace of the same size) against the oracle on sampled sequences on
 both sides of the 2^29-element (4 GiB) and 2^31-element boundaries, where a
 32-bit byte or element offset would wrap.  About 35 GB (leaf) and 52 GB
-(chain) of device memory.
-Bounds: FP32 2e-6 log2 N relative L2 per sampled transform.
+(chain) of device memory; the FP64 Bluestein case about 103 GB.
+Bounds: FP32 2e-6 log2 N, FP64 1e-14 log2 N relative L2 per sampled transform.
 """
 import numpy as np
 import pytest
@@ -52,25 +52,28 @@ def _rows(n, batch, elems_per_row):
     return sorted(rows)
 
 
-def test_bluestein_past_2_31_elements(gpu):
-    """N = 10007 (Bluestein, one workgroup's LDS per sequence), batch 214601."""
+@pytest.mark.parametrize("dbl", [False, True], ids=["c2c", "z2z"])
+def test_bluestein_past_2_31_elements(gpu, dbl):
+    """N = 10007 (Bluestein in LDS: the whole convolution in one workgroup for
+    FP32, two halves with a parked first half for FP64), batch 214601."""
     import torch
 
     n, batch = 10007, 214601
+    t = ff.Type.Z2Z if dbl else ff.Type.C2C
     assert n * batch > 1 << 31
     gen = torch.Generator(device=gpu)
-    gen.manual_seed(n)
-    x = torch.randn(n * batch, dtype=torch.complex64, device=gpu, generator=gen)
+    gen.manual_seed(n + dbl)
+    x = torch.randn(n * batch, dtype=torch.complex128 if dbl else torch.complex64, device=gpu, generator=gen)
     y = torch.empty_like(x)
     try:
-        with ff.Plan.one_d(n, ff.Type.C2C, batch) as p:
+        with ff.Plan.one_d(n, t, batch) as p:
             assert "Bluestein" in p.description()
             p.set_stream(stream_of(gpu))
-            p.c2c(x.data_ptr(), y.data_ptr(), ff.FORWARD)
+            (p.z2z if dbl else p.c2c)(x.data_ptr(), y.data_ptr(), ff.FORWARD)
             sync(gpu)
         for r in _rows(n, batch, n):
             xr = host(x[r * n:(r + 1) * n]).astype(np.complex128)
-            check_accuracy(host(y[r * n:(r + 1) * n])[None], ora.fft(xr[None]), ff.Type.C2C, n, row_len=n,
+            check_accuracy(host(y[r * n:(r + 1) * n])[None], ora.fft(xr[None]), t, n, row_len=n,
                            what=f"N={n} sequence {r}")
     finally:
         del x, y
