@@ -109,3 +109,31 @@ def test_real_past_2_31_elements(gpu):
     finally:
         del x, X, z
         torch.cuda.empty_cache()
+
+
+def test_2d_batch_past_2_31_elements(gpu):
+    """Rank-2 C2C 2048 x 2048, batch 513 (flagfftPlanMany): rows then the
+    column pass over column-interleaved units, batch entries sampled either
+    side of the 2^29- and 2^31-element marks."""
+    import torch
+
+    n0 = n1 = 2048
+    batch = 513
+    m = n0 * n1
+    assert m * batch > 1 << 31
+    gen = torch.Generator(device=gpu)
+    gen.manual_seed(m + batch)
+    x = torch.randn(m * batch, dtype=torch.complex64, device=gpu, generator=gen)
+    y = torch.empty_like(x)
+    try:
+        with ff.Plan.many(2, [n0, n1], None, 1, m, None, 1, m, ff.Type.C2C, batch) as p:
+            p.set_stream(stream_of(gpu))
+            p.c2c(x.data_ptr(), y.data_ptr(), ff.FORWARD)
+            sync(gpu)
+        for r in _rows(m, batch, m):
+            xr = host(x[r * m:(r + 1) * m]).astype(np.complex128).reshape(1, n0, n1)
+            check_accuracy(host(y[r * m:(r + 1) * m]).reshape(1, n0, n1), ora.fft2(xr), ff.Type.C2C, m, row_len=m,
+                           what=f"2D entry {r}")
+    finally:
+        del x, y
+        torch.cuda.empty_cache()
