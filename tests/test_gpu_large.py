@@ -137,3 +137,60 @@ def test_2d_batch_past_2_31_elements(gpu):
     finally:
         del x, y
         torch.cuda.empty_cache()
+
+
+def _closed_form_err(y, n, fn, chunk=1 << 26):
+    """Relative L2 of y against fn(k) over k < n, FP64 on the device in chunks."""
+    import torch
+
+    num = den = 0.0
+    for s in range(0, n, chunk):
+        k = torch.arange(s, min(n, s + chunk), dtype=torch.int64, device=y.device)
+        ref = fn(k, s)
+        d = y[s:s + k.numel()].to(torch.complex128) - ref
+        num += float((d.real * d.real + d.imag * d.imag).sum())
+        den += float((ref.real * ref.real + ref.imag * ref.imag).sum())
+    return (num / den) ** 0.5
+
+
+@pytest.mark.parametrize("logn", [28, 30])
+def test_longest_chains(gpu, logn):
+    """N = 2^28 and 2^30 (three-pass chains, one sequence of 2 / 8 GiB): the
+    spectrum of an impulse at j is w_N^(jk) (every inter-pass twiddle enters
+    it), and forward then inverse of N(0,1) data is N x.  The expected values
+    are closed forms in FP64 on the device; the FP64 oracle would need minutes
+    at this length."""
+    import math
+
+    import torch
+
+    n = 1 << logn
+    bound = 2e-6 * logn
+    j = 123456789 % n
+    x = torch.zeros(n, dtype=torch.complex64, device=gpu)
+    x[j] = 1
+    y = torch.empty_like(x)
+    try:
+        with ff.Plan.one_d(n, ff.Type.C2C, 1) as p:
+            assert "passes=3" in p.description()
+            p.set_stream(stream_of(gpu))
+            p.c2c(x.data_ptr(), y.data_ptr(), ff.FORWARD)
+            sync(gpu)
+
+            def impulse(k, s):
+                ang = ((j * k) % n).to(torch.float64) * (-2.0 * math.pi / n)
+                return torch.complex(torch.cos(ang), torch.sin(ang))
+
+            e = _closed_form_err(y, n, impulse)
+            assert e <= bound, f"impulse at {j}: rel L2 {e:.3e} > {bound:.3e}"
+            gen = torch.Generator(device=gpu)
+            gen.manual_seed(logn)
+            x = torch.randn(n, dtype=torch.complex64, device=gpu, generator=gen)
+            p.c2c(x.data_ptr(), y.data_ptr(), ff.FORWARD)
+            p.c2c(y.data_ptr(), y.data_ptr(), ff.INVERSE)
+            sync(gpu)
+            e = _closed_form_err(y, n, lambda k, s: x[s:s + k.numel()].to(torch.complex128) * n)
+            assert e <= 2 * bound, f"roundtrip: rel L2 {e:.3e} > {2 * bound:.3e}"
+    finally:
+        del x, y
+        torch.cuda.empty_cache()
