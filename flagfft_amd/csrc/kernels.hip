@@ -744,7 +744,9 @@ template <typename T>
 __global__ void k_gen_roots(V2<T> *out, int64_t count, int64_t n, int64_t mult) {
   for (int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; j < count;
        j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t r = (j % n) * (mult % n) % n;
+    // residues below n <= 2^32, so their product fits 64 unsigned bits
+    const uint64_t un = static_cast<uint64_t>(n);
+    const uint64_t r = (static_cast<uint64_t>(j) % un) * (static_cast<uint64_t>(mult) % un) % un;
     double s, c;
     sincospi(-2.0 * static_cast<double>(r) / static_cast<double>(n), &s, &c);
     out[j] = V2<T>{static_cast<T>(c), static_cast<T>(s)};
