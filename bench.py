@@ -1,31 +1,38 @@
 """Benchmark of the FFT execution path behind include/flagfft.h on one MI355X.
 
-Default workload (arbitrary-length 1D, one "step" = one pass of the hot path
-over one batch of synthetic input):
-  C2C f32 and Z2Z f64 forward at N=10007 (prime) and N=10935 = 3^7*5, batch 64, N(0,1);
-  R2C->C2R f32 and D2Z->Z2D f64 at N=6000, batch 128, U[-1,1].
-Every transform goes through the C ABI (libflagfft.so).
-Inputs rotate over buffer sets larger than 2x the 256 MB Infinity Cache so
-the timed region reads HBM.
+Headline workload (the large C2C/R2C configuration; one "step" = one pass of
+the hot path over one batch of synthetic input, every transform through the
+C ABI of libflagfft.so):
+  C2C f32 forward N=2^24 x 1 and N=2^20 x 16 (re, im iid N(0,1));
+  2D R2C -> C2R f32 4096 x 4096 (real inputs iid N(0,1));
+  2D Z2Z f64 forward 2048 x 2048 (re, im iid N(0,1)).
+Inputs rotate over buffer sets larger than 2x the 256 MB Infinity Cache so the
+timed region reads HBM.
 
 value          algorithmic HBM GB/s of the whole step: sum over transforms of
-               batch * (input bytes + output bytes), divided by wall time; the
-               transform families (independent) are spread longest-first by
-               their device time alone over 1 to 4 HIP streams (4 hardware
-               queues per process), the count chosen by a short calibration
-               region before the timed one; config.single_stream_gbps is the
-               same step on one stream
+               input bytes + output bytes, divided by wall time (device
+               synchronised on both sides of the timed region); the transform
+               families are spread longest-first by their device time over 1
+               to 4 HIP streams (4 hardware queues per process), the count
+               chosen by a short calibration region before the timed one
+config.pass_bytes_gbps
+               the same region counted as passes x bytes (every launch's read
+               + write, workspaces included): the north-star accounting
 roofline       dominant kernel (by total device time): algorithmic bytes per
                launch / mean launch duration from HIP events recorded on the
                plan's stream around every launch of a second timed region of
-               the same K steps (the first, event-free region gives value);
-               traffic = counter-measured HBM bytes per launch (committed
-               rocprofv3 summary), traffic_gbps / traffic_frac = those bytes
-               over the same duration
+               the same K steps; traffic = counter-measured HBM bytes per
+               launch (committed rocprofv3 summary)
 cpu_baseline   the oracle's fast FFT (oracle/oracle.c) on the host cores over
-               a bounded sample of the same step
+               the same step (median of 10 repetitions)
+secondary      the other configurations under their own keys: the
+               arbitrary-length 1D mix, C2C f32 4096 x 256 forward + inverse,
+               the precision form (1024 x 65536 vector / FP16 hi+lo / BF16
+               hi+lo with their errors against the FP64 oracle) and the
+               serving leg (Poisson stream, batched vs plain launches)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--suite]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--headline-only] [--serve-requests R] [--suite]
 """
 from __future__ import annotations
 
@@ -50,22 +57,26 @@ def log(*a):
 
 
 class Case:
-    """One transform family of the workload: plans, rotating device buffers, one exec."""
+    """One transform family of a workload: plans, rotating device buffers, one exec.
 
-    def __init__(self, name, kind, n, batch, dbl, dev, ff, torch, compute=0):
+    kind: "c2c" (forward), "c2c_rt" (forward then inverse), "real" (R2C then C2R).
+    n: an int (1D, `batch` rows) or a tuple (2D / 3D row-major, one transform)."""
+
+    def __init__(self, name, kind, n, batch, dbl, dev, ff, torch, compute=0, real_dist=None):
         self.name, self.kind, self.n, self.batch, self.dbl = name, kind, n, batch, dbl
         self.compute = compute
         self.ff, self.torch, self.dev = ff, torch, dev
+        # real inputs: iid N(0,1) for the 2D / 3D transforms, U[-1,1] for the 1D ones
+        self.real_dist = real_dist or ("normal" if isinstance(n, tuple) else "uniform")
         cdt = torch.complex128 if dbl else torch.complex64
         rdt = torch.float64 if dbl else torch.float32
         esz = 16 if dbl else 8
         T = ff.Type
+        F, I = ff.FORWARD, ff.INVERSE
         if isinstance(n, tuple):  # 2D / 3D (row-major, last axis contiguous), one transform per exec
             lead, nl = n[:-1], n[-1]
             h = nl // 2 + 1
-            outer = 1
-            for v in lead:
-                outer *= v
+            outer = int(np.prod(lead))
             self.batch = 1
 
             def plan(t):
@@ -73,25 +84,32 @@ class Case:
 
             if kind == "c2c":
                 self.plans = [plan(T.Z2Z if dbl else T.C2C)]
+                self.profile = [(0, F)]
                 self.alg_bytes = outer * nl * esz * 2
                 self.shapes = [(n, cdt), (n, cdt)]
             else:
                 self.plans = [plan(T.D2Z if dbl else T.R2C), plan(T.Z2D if dbl else T.C2R)]
+                self.profile = [(0, F), (1, I)]
                 self.alg_bytes = 2 * (outer * nl * esz // 2 + outer * h * esz)
                 self.shapes = [(n, rdt), (lead + (h,), cdt), (n, rdt)]
             self.sets = []
             return
         h = n // 2 + 1
-        if kind == "c2c":
+        if kind in ("c2c", "c2c_rt"):
             self.plans = [ff.Plan.one_d_compute(n, T.Z2Z if dbl else T.C2C, batch, compute)]
-            self.alg_bytes = batch * n * esz * 2
-            self.shapes = [((batch, n), cdt), ((batch, n), cdt)]
+            self.profile = [(0, F)] if kind == "c2c" else [(0, F), (0, I)]
+            self.alg_bytes = batch * n * esz * 2 * (1 if kind == "c2c" else 2)
+            self.shapes = [((batch, n), cdt), ((batch, n), cdt)] + ([((batch, n), cdt)] if kind == "c2c_rt" else [])
         else:  # real roundtrip R2C -> C2R
             self.plans = [ff.Plan.one_d(n, T.D2Z if dbl else T.R2C, batch),
                           ff.Plan.one_d(n, T.Z2D if dbl else T.C2R, batch)]
+            self.profile = [(0, F), (1, I)]
             self.alg_bytes = 2 * batch * (n * esz // 2 + h * esz)
             self.shapes = [((batch, n), rdt), ((batch, h), cdt), ((batch, n), rdt)]
         self.sets = []
+
+    def transforms(self):
+        return self.batch * (1 if self.kind == "c2c" else 2)
 
     def footprint(self):
         return sum(int(np.prod(s)) * self.torch.empty((), dtype=d).element_size() for s, d in self.shapes)
@@ -107,6 +125,8 @@ class Case:
                     if dt.is_complex:
                         t = torch.randn(shape, dtype=dt, device=self.dev, generator=g)  # re, im iid N(0, 1/2)
                         t = t * (2.0 ** 0.5)                                             # -> iid N(0, 1)
+                    elif self.real_dist == "normal":
+                        t = torch.randn(shape, dtype=dt, device=self.dev, generator=g)  # N(0, 1)
                     else:
                         t = torch.rand(shape, dtype=dt, device=self.dev, generator=g) * 2 - 1  # U[-1, 1)
                 else:
@@ -121,32 +141,86 @@ class Case:
     def run(self, step):
         b = self.sets[step % len(self.sets)]
         p = self.plans
-        if self.kind == "c2c":
-            (p[0].z2z if self.dbl else p[0].c2c)(b[0].data_ptr(), b[1].data_ptr(), self.ff.FORWARD)
+        if self.kind in ("c2c", "c2c_rt"):
+            ex = p[0].z2z if self.dbl else p[0].c2c
+            ex(b[0].data_ptr(), b[1].data_ptr(), self.ff.FORWARD)
+            if self.kind == "c2c_rt":
+                ex(b[1].data_ptr(), b[2].data_ptr(), self.ff.INVERSE)
         else:
             (p[0].d2z if self.dbl else p[0].r2c)(b[0].data_ptr(), b[1].data_ptr())
             (p[1].z2d if self.dbl else p[1].c2r)(b[1].data_ptr(), b[2].data_ptr())
 
-    def cpu_step(self, ora, nthreads, frac):
-        """Oracle fast FFT over a fraction of the batch; returns (seconds, algorithmic bytes)."""
-        nb = max(1, int(self.batch * frac))
-        rng = np.random.default_rng(self.n)
-        if self.kind == "c2c":
-            x = (rng.standard_normal((nb, self.n)) + 1j * rng.standard_normal((nb, self.n)))
-            x = x.astype(np.complex128 if self.dbl else np.complex64)
-            t0 = time.perf_counter()
-            (ora.fft if self.dbl else ora.fft_f32)(x, -1, nthreads)
-            dt = time.perf_counter() - t0
+    # ---- the same step on the host cores (the oracle's fast FFT) -------------
+    def cpu_prepare(self, nthreads):
+        """Host input of the whole step and the tables a CPU plan would build once."""
+        rng = np.random.default_rng(abs(hash(self.name)) % (1 << 32))
+        shape = self.n if isinstance(self.n, tuple) else (self.batch, self.n)
+        if self.kind == "real":
+            x = rng.standard_normal(shape) if self.real_dist == "normal" else rng.uniform(-1, 1, shape)
+            self.cpu_x = x.astype(np.float64 if self.dbl else np.float32)
         else:
-            x = rng.uniform(-1, 1, (nb, self.n)).astype(np.float64 if self.dbl else np.float32)
-            t0 = time.perf_counter()
-            X = (ora.rfft if self.dbl else ora.rfft_f32)(x, nthreads)
-            (ora.irfft if self.dbl else ora.irfft_f32)(X, self.n, nthreads)
-            dt = time.perf_counter() - t0
-        return dt, self.alg_bytes * nb / self.batch
+            x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            self.cpu_x = x.astype(np.complex128 if self.dbl else np.complex64)
+        self.cpu_tw = None
+        if not isinstance(self.n, tuple) and self.batch < nthreads and self.n >= (1 << 16) and self.n & (self.n - 1) == 0:
+            # batch below the thread count: four-step over sqrt(N) x sqrt(N) so every core works
+            lg = self.n.bit_length() - 1
+            n1, n2 = 1 << (lg // 2), 1 << (lg - lg // 2)
+            b = np.arange(n2)[:, None]
+            k = np.arange(n1)[None, :]
+            tw = np.exp(-2j * np.pi * ((b * k) % self.n) / self.n)
+            self.cpu_tw = (n1, n2, tw.astype(np.complex128 if self.dbl else np.complex64))
+
+    def _c2c_rows(self, ora, x, sign, nthreads):
+        return (ora.fft if self.dbl else ora.fft_f32)(x, sign, nthreads)
+
+    def _c2c_1d(self, ora, x, sign, nthreads):
+        if self.cpu_tw is None:
+            return self._c2c_rows(ora, x, sign, nthreads)
+        n1, n2, tw = self.cpu_tw
+        out = np.empty_like(x)
+        for r in range(x.shape[0]):  # X[k1 + n1 k2] = sum_b w_N^(b k1) w_n2^(b k2) sum_a x[n2 a + b] w_n1^(a k1)
+            B = np.ascontiguousarray(x[r].reshape(n1, n2).T)
+            C = self._c2c_rows(ora, B, sign, nthreads)
+            C *= tw if sign < 0 else np.conj(tw)
+            E = self._c2c_rows(ora, np.ascontiguousarray(C.T), sign, nthreads)
+            out[r] = np.ascontiguousarray(E.T).reshape(-1)
+        return out
+
+    def cpu_run(self, ora, nthreads):
+        x = self.cpu_x
+        if isinstance(self.n, tuple):
+            if len(self.n) != 2:
+                raise NotImplementedError("CPU baseline of rank-3 transforms")
+            if self.kind == "c2c":
+                if self.dbl:
+                    return ora.fft2(x, -1, nthreads)
+                y = self._c2c_rows(ora, x, -1, nthreads)
+                return np.ascontiguousarray(self._c2c_rows(ora, np.ascontiguousarray(y.T), -1, nthreads).T)
+            rf, irf = (ora.rfft, ora.irfft) if self.dbl else (ora.rfft_f32, ora.irfft_f32)
+            X = rf(x, nthreads)  # rows, then columns over a transpose
+            X = self._c2c_rows(ora, np.ascontiguousarray(X.T), -1, nthreads)
+            X = self._c2c_rows(ora, X, 1, nthreads)  # inverse: columns, then rows
+            return irf(np.ascontiguousarray(X.T), self.n[-1], nthreads)
+        if self.kind == "real":
+            rf, irf = (ora.rfft, ora.irfft) if self.dbl else (ora.rfft_f32, ora.irfft_f32)
+            return irf(rf(x, nthreads), self.n, nthreads)
+        y = self._c2c_1d(ora, x, -1, nthreads)
+        if self.kind == "c2c_rt":
+            y = self._c2c_1d(ora, y, 1, nthreads)
+        return y
 
 
-WORKLOAD = [
+WORKLOAD = [  # headline: C2C/R2C at N = 2^20 - 2^24 and the 2D transforms
+    ("C2C f32 N=2^24 x1", "c2c", 1 << 24, 1, False),
+    ("C2C f32 N=2^20 x16", "c2c", 1 << 20, 16, False),
+    ("2D R2C->C2R f32 4096x4096", "real", (4096, 4096), 1, False),
+    ("2D Z2Z f64 2048x2048", "c2c", (2048, 2048), 1, True),
+]
+WORKLOAD_NAME = ("large C2C/R2C: C2C f32 forward N=2^24 x1 and N=2^20 x16, 2D R2C->C2R f32 4096x4096, "
+                 "2D Z2Z f64 forward 2048x2048; N(0,1) inputs")
+
+ARBITRARY = [  # arbitrary-length 1D (Rader/Bluestein primes, mixed radix, real roundtrips)
     ("C2C f32 N=10007 x64", "c2c", 10007, 64, False),
     ("C2C f32 N=10935 x64", "c2c", 10935, 64, False),
     ("Z2Z f64 N=10007 x64", "c2c", 10007, 64, True),
@@ -155,16 +229,15 @@ WORKLOAD = [
     ("D2Z->Z2D f64 N=6000 x128", "real", 6000, 128, True),
 ]
 
-SUITE = [  # the other configurations, reported with --suite (not the headline line)
-    [("C2C f32 N=4096 x256", "c2c", 4096, 256, False)],
-    [("C2C f32 N=2^20 x16", "c2c", 1 << 20, 16, False)],
-    [("C2C f32 N=2^24 x1", "c2c", 1 << 24, 1, False)],
+SMALL = [("C2C f32 N=4096 x256 forward+inverse", "c2c_rt", 4096, 256, False)]
+
+PRECISION = [  # (name, compute mode)
+    ("vector FP32", 0), ("FP16 hi+lo matrix cores", 2), ("BF16 hi+lo matrix cores", 1),
+]
+
+SUITE = [  # further configurations, reported with --suite (stderr)
     [("C2C f32 N=1024 x65536", "c2c", 1024, 65536, False)],
-    [("C2C f32 N=1024 x65536 FP16 hi+lo matrix cores", "c2c", 1024, 65536, False, 2)],
-    [("C2C f32 N=1024 x65536 BF16 hi+lo matrix cores", "c2c", 1024, 65536, False, 1)],
     [("R2C->C2R f32 N=2^20 x16", "real", 1 << 20, 16, False)],
-    [("2D R2C->C2R f32 4096x4096", "real", (4096, 4096), 1, False)],
-    [("2D Z2Z f64 2048x2048", "c2c", (2048, 2048), 1, True)],
     [("3D C2C f32 256x256x256", "c2c", (256, 256, 256), 1, False)],
     [("3D R2C->C2R f32 256x256x256", "real", (256, 256, 256), 1, False)],
 ]
@@ -304,10 +377,9 @@ def measure(cases, steps, warmup, torch, ff, stream, case_streams, slot):
     per_case = {}
     for c in cases:
         case_ms = 0.0
-        for pi, p in enumerate(c.plans):
-            direction = ff.FORWARD if (c.kind == "c2c" or pi == 0) else ff.INVERSE
-            for label, ms, calls, nbytes in ff.get_profile(p.handle, direction):
-                key = (c.name, pi, label)
+        for pi, direction in c.profile:
+            for label, ms, calls, nbytes in ff.get_profile(c.plans[pi].handle, direction):
+                key = (c.name, pi, direction, label)
                 launches[key] = (ms, calls, nbytes)
                 case_ms += ms
         per_case[c.name] = {"alg_bytes": c.alg_bytes, "device_ms_per_exec": case_ms / steps,
@@ -371,7 +443,7 @@ def write_traffic(args):
 def roofline_of(launches):
     """Dominant kernel symbol by total device time; achieved = bytes / mean duration."""
     by_sym = {}
-    for (case, pi, label), (ms, calls, nbytes) in launches.items():
+    for (case, pi, direction, label), (ms, calls, nbytes) in launches.items():
         sym = kernel_symbol(label)
         d = by_sym.setdefault(sym, {"ms": 0.0, "calls": 0, "bytes": 0, "labels": set()})
         d["ms"] += ms
@@ -394,31 +466,54 @@ def roofline_of(launches):
         # close the kernel's actual traffic comes to the HBM roof
         roof["traffic_gbps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
         roof["traffic_frac"] = round(roof["traffic_gbps"] / HBM_PEAK_GBS, 4)
+    # "bound" is the roof the kernel's arithmetic intensity puts it under (every
+    # FFT kernel here is below the FP32 vector ridge); "limiter" says whether
+    # the measured rate actually reaches that roof
+    seen = roof.get("traffic_frac", roof["frac"])
+    roof["limiter"] = "hbm" if seen >= 0.6 else "not hbm: issue or latency bound below 60% of the HBM roof"
     return roof, {k: {"ms": round(v["ms"], 3), "calls": v["calls"], "avg_us": round(v["ms"] / max(v["calls"], 1) * 1e3, 2)}
         for k, v in by_sym.items()}
 
 
-def cpu_baseline(cases, budget_s=12.0):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(cases, reps: int = 10):
+    """The same step through the oracle's fast FFT on the host cores (FP32 for
+    the f32 cases, FP64 for the f64 ones): one untimed repetition, then the
+    median of `reps` timed ones."""
     from oracle import oracle as ora
 
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     nthreads = max(1, min(nthreads, os.cpu_count() or 1))
-    frac = 0.25
-    total_t = 0.0
-    total_b = 0.0
-    reps = 0
-    t_start = time.perf_counter()
-    while reps < 1 or time.perf_counter() - t_start < budget_s:
+    for c in cases:
+        c.cpu_prepare(nthreads)
+    alg = sum(c.alg_bytes for c in cases)
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
         for c in cases:
-            dt, b = c.cpu_step(ora, nthreads, frac)
-            total_t += dt
-            total_b += b
-        reps += 1
-        if reps >= 200:
-            break
-    return {"value": round(total_b / total_t / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
-            "sample": f"{reps} x the same step with each batch cut to {int(frac * 100)}% "
-                      f"(oracle fast FFT, FP32 for f32 cases, FP64 for f64 cases, {total_t:.1f} s wall)"}
+            c.cpu_run(ora, nthreads)
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(alg / med / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"the whole step ({alg / 2**20:.0f} MiB of algorithmic bytes) on {nthreads} host threads, "
+                      f"median of {reps} repetitions after one untimed ({med * 1e3:.0f} ms per step, "
+                      f"{sum(times):.1f} s timed); oracle fast FFT, FP32 for f32 cases, FP64 for f64 cases; "
+                      "batch-1 2^24 as a threaded four-step, 2D as rows then columns over a transpose",
+            "ms_per_step": round(med * 1e3, 1)}
 
 
 def _is_prime(n: int) -> bool:
@@ -524,21 +619,52 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
     return out
 
 
-def accuracy_sample(case, torch, rows: int = 64):
-    """Relative L2 error of the first rows of one exec against the FP64 oracle."""
+def sample_rows(batch: int, per_part: int = 16):
+    """Rows from the start, the middle and the end of a batch (every part of a
+    kernel's grid and of a persistent kernel's loop)."""
+    rows = set()
+    for start in (0, batch // 4, batch // 2, batch - per_part):
+        rows.update(range(max(0, start), min(batch, max(0, start) + per_part)))
+    return np.array(sorted(rows))
+
+
+def accuracy(case, torch):
+    """Relative L2 errors of one exec of `case` (buffer set 0) against the FP64
+    oracle: the forward output, and for round trips the second transform's
+    output against n x the input (the inverse is unnormalised)."""
     from oracle import oracle as ora
 
     b = case.sets[0]
     case.run(0)
     torch.cuda.synchronize()
-    if isinstance(case.n, tuple):  # 2D / 3D: the whole transform
-        x = b[0].cpu().numpy().astype(np.complex128)
-        y = b[1].cpu().numpy()
-        ref = ora.fft2(x[None]) if len(case.n) == 2 else ora.fft3(x[None])
-        return float(f"{ora.rel_l2(y.reshape(1, -1), ref.reshape(1, -1))[0]:.3e}")
-    x = b[0][:rows].cpu().numpy().astype(np.complex128)
-    y = b[1][:rows].cpu().numpy()
-    return float(f"{ora.rel_l2(y, ora.fft(x))[0]:.3e}")
+    out = {}
+
+    def err(y, ref):
+        return float(f"{ora.rel_l2(y, ref, y.shape[-1])[0]:.3e}")
+
+    if isinstance(case.n, tuple):
+        if len(case.n) != 2:
+            return out
+        x = b[0].cpu().numpy()
+        npts = case.n[0] * case.n[1]
+        if case.kind == "c2c":
+            out["forward"] = err(b[1].cpu().numpy().reshape(1, -1), ora.fft2(x.astype(np.complex128)).reshape(1, -1))
+        else:
+            out["forward"] = err(b[1].cpu().numpy().reshape(1, -1), ora.rfft2(x.astype(np.float64)).reshape(1, -1))
+            out["roundtrip"] = err(b[2].cpu().numpy().reshape(1, -1), npts * x.astype(np.float64).reshape(1, -1))
+        return out
+    rows = sample_rows(case.batch)
+    rt = torch.as_tensor(rows, device=case.dev)
+    x = b[0].index_select(0, rt).cpu().numpy()
+    if case.kind == "real":
+        out["forward"] = err(b[1].index_select(0, rt).cpu().numpy(), ora.rfft(x.astype(np.float64)))
+        out["roundtrip"] = err(b[2].index_select(0, rt).cpu().numpy(), case.n * x.astype(np.float64))
+    else:
+        out["forward"] = err(b[1].index_select(0, rt).cpu().numpy(), ora.fft(x.astype(np.complex128)))
+        if case.kind == "c2c_rt":
+            out["roundtrip"] = err(b[2].index_select(0, rt).cpu().numpy(), case.n * x.astype(np.complex128))
+    out["rows_checked"] = int(len(rows))
+    return out
 
 
 def run_workload(spec, args, torch, ff, dev, stream, seed):
@@ -568,13 +694,28 @@ def run_workload(spec, args, torch, ff, dev, stream, seed):
                    "per_case": per_case, "buffer_sets": nsets, "footprint_mb": round(nsets * footprint / 2**20)}
 
 
+def leg_summary(res):
+    """The figures of one measured workload other than the headline."""
+    return {"gbps": round(res["value"], 1), "frac_of_peak": round(res["value"] / HBM_PEAK_GBS, 4),
+            "pass_bytes_gbps": round(res["pass_gbps"], 1), "us_per_step": round(res["ms_per_step"] * 1e3, 2),
+            "streams": res["streams"], "roofline": res["roofline"],
+            "per_case": per_case_summary(res)}
+
+
+def per_case_summary(res):
+    return {k: {"gbps": round(v["gbps"], 1) if v["gbps"] else None,
+                "device_us": round(v["device_ms_per_exec"] * 1e3, 2)} for k, v in res["per_case"].items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--suite", action="store_true", help="also measure the other configurations (stderr)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64-oracle error reports")
+    ap.add_argument("--headline-only", action="store_true", help="measure the headline workload only")
+    ap.add_argument("--suite", action="store_true", help="also measure further configurations (stderr)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving leg (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes 20x "
                     "(counter calibration run under rocprofv3)")
@@ -607,16 +748,48 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
-    cases, res = run_workload(WORKLOAD, args, torch, ff, dev, stream, 1234)
 
-    cpu = None if args.no_cpu else cpu_baseline(cases)
-    transforms_per_step = sum(c.batch * (1 if c.kind == "c2c" else 2) for c in cases)
-    serving = None
-    if args.serve_requests > 0:
-        del cases
+    def release(cases):
+        for c in cases:
+            for p in c.plans:
+                p.destroy()
+            c.sets.clear()
+        torch.cuda.synchronize()
         torch.cuda.empty_cache()
-        serving = serving_leg(torch, ff, dev, args.serve_requests)
-        cases = []
+
+    log("headline workload: " + WORKLOAD_NAME)
+    cases, res = run_workload(WORKLOAD, args, torch, ff, dev, stream, 1234)
+    errors = {} if args.no_accuracy else {c.name: accuracy(c, torch) for c in cases}
+    log("headline measured; accuracy " + json.dumps(errors))
+    cpu = None if args.no_cpu else cpu_baseline(cases)
+    transforms_per_step = sum(c.transforms() for c in cases)
+    release(cases)
+
+    secondary = {}
+    if not args.headline_only:
+        for key, spec, seed in (("arbitrary_length_1d", ARBITRARY, 2001), ("c2c_f32_4096x256_roundtrip", SMALL, 2002)):
+            log(f"secondary workload: {key}")
+            cases, r = run_workload(spec, args, torch, ff, dev, stream, seed)
+            secondary[key] = leg_summary(r)
+            if not args.no_accuracy:
+                secondary[key]["rel_l2_vs_fp64"] = {c.name: accuracy(c, torch) for c in cases}
+            release(cases)
+        prec = {}
+        for name, mode in PRECISION:
+            log(f"precision form: {name}")
+            cases, r = run_workload([(f"C2C f32 N=1024 x65536 {name}", "c2c", 1024, 65536, False, mode)], args, torch,
+                                    ff, dev, stream, 2003)
+            prec[name] = {"gbps": round(r["value"], 1), "frac_of_peak": round(r["value"] / HBM_PEAK_GBS, 4),
+                          "us_per_exec": round(r["ms_per_step"] * 1e3, 2),
+                          "kernel": r["roofline"]["kernel"], "kernel_us": r["roofline"]["avg_launch_us"]}
+            if not args.no_accuracy:
+                prec[name]["rel_l2_vs_fp64"] = accuracy(cases[0], torch)
+            release(cases)
+        secondary["precision_form_1024x65536"] = prec
+        if args.serve_requests > 0:
+            log("serving leg")
+            secondary["serving"] = serving_leg(torch, ff, dev, args.serve_requests)
+
     line = {
         "metric": METRIC,
         "value": round(res["value"], 2),
@@ -631,12 +804,12 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "arbitrary-length 1D: C2C f32 + Z2Z f64 forward at N=10007 (prime) and N=10935=3^7*5, "
-                        "batch 64, N(0,1); R2C->C2R f32 + D2Z->Z2D f64 at N=6000, batch 128, U[-1,1]",
+            "workload": WORKLOAD_NAME,
             "transforms_per_step": transforms_per_step,
             "alg_bytes_per_step": res["alg_bytes_per_step"],
             "hbm_frac_of_peak": round(res["value"] / HBM_PEAK_GBS, 4),
             "pass_bytes_gbps": round(res["pass_gbps"], 2),
+            "pass_bytes_frac_of_peak": round(res["pass_gbps"] / HBM_PEAK_GBS, 4),
             "input_rotation": f"{res['buffer_sets']} buffer sets, {res['footprint_mb']} MiB",
             "parallelism": f"single GPU, transform families over {res['streams']} HIP streams "
                            f"(count calibrated over 1-{hw_queues()}), balanced by device time",
@@ -645,20 +818,19 @@ def main():
         },
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
-        "serving": serving,
-        "per_case": {k: {"gbps": round(v["gbps"], 1) if v["gbps"] else None,
-                         "device_us": round(v["device_ms_per_exec"] * 1e3, 2)} for k, v in res["per_case"].items()},
+        "rel_l2_vs_fp64": errors,
+        "per_case": per_case_summary(res),
         "kernels": res["kernels"],
+        "secondary": secondary,
     }
     if args.suite:
         for spec in SUITE:
-            del cases
-            torch.cuda.empty_cache()
             cases, r = run_workload(spec, args, torch, ff, dev, stream, 99)
-            err = accuracy_sample(cases[0], torch) if cases[0].kind == "c2c" else None
+            err = None if args.no_accuracy else accuracy(cases[0], torch)
             log(json.dumps({"suite": spec[0][0], "gbps": round(r["value"], 1), "rel_l2_vs_fp64": err,
                             "frac": round(r["value"] / HBM_PEAK_GBS, 3), "us_per_exec": round(r["ms_per_step"] * 1e3, 2),
-                            "roofline": r["roofline"]}))
+                            "pass_bytes_gbps": round(r["pass_gbps"], 1), "roofline": r["roofline"]}))
+            release(cases)
     print(json.dumps(line), flush=True)
 
 

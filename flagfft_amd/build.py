@@ -31,6 +31,30 @@ COMMON_FLAGS = [
 ]
 
 
+# Sources whose hashes fingerprint the tuned-plan rows: a row applies only to
+# the planner and kernels it was measured with.
+PLANNER_SOURCES = ["planner.cpp", "planner.hpp", "exec.cpp", "exec.hpp"]
+KERNEL_SOURCES = [s for s in SOURCES if s.endswith(".hip")] + [
+    "fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp"]
+
+
+def source_hash(names) -> str:
+    """First 16 hex digits of the SHA-256 over the named csrc files (name and bytes)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in sorted(names):
+        h.update(name.encode() + b"\0")
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def fingerprint_flags():
+    return [f'-DFLAGFFT_PLANNER_HASH="{source_hash(PLANNER_SOURCES)}"',
+            f'-DFLAGFFT_KERNEL_HASH="{source_hash(KERNEL_SOURCES)}"']
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -49,11 +73,13 @@ def _compile(src: str, force: bool) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
     if not force and os.path.exists(obj):
-        if os.path.getmtime(obj) >= max(os.path.getmtime(path), _newest_dep()):
+        deps = [path] + ([os.path.join(CSRC, n) for n in PLANNER_SOURCES + KERNEL_SOURCES] if src == "tunedb.cpp" else [])
+        if os.path.getmtime(obj) >= max(max(os.path.getmtime(d) for d in deps), _newest_dep()):
             return obj
-    cmd = [hipcc()] + COMMON_FLAGS + ["-c", path, "-o", obj]
+    flags = COMMON_FLAGS + (fingerprint_flags() if src == "tunedb.cpp" else [])
+    cmd = [hipcc()] + flags + ["-c", path, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [hipcc()] + COMMON_FLAGS + ["-x", "hip", "-c", path, "-o", obj]
+        cmd = [hipcc()] + flags + ["-x", "hip", "-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

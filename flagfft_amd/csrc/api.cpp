@@ -7,6 +7,7 @@
 //   Exec* / SetStream / Destroy / GetPlanDescription  src/exec/c_api_exec.cpp:5-236
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -264,6 +265,17 @@ flagfftResult build_plan(flagfftHandle *out, const Desc &d) {
   }
 }
 
+// The executable one Exec of `direction` runs (real plans have one), with
+// the in-place variant when the buffers are one and the plan has a hazard.
+Executable *select_exec(PlanImpl &p, const void *in, const void *out, int direction) {
+  Executable *ex = nullptr;
+  if (p.desc.real_fwd) ex = p.fwd.get();
+  else if (p.desc.real_inv) ex = p.inv.get();
+  else ex = direction == FLAGFFT_INVERSE ? p.inv.get() : p.fwd.get();
+  if (ex != nullptr && in == out && ex->inplace_hazard) ex = ex == p.fwd.get() ? p.fwd_ip.get() : p.inv_ip.get();
+  return ex;
+}
+
 flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagfftType expect, int direction,
                           bool has_direction) {
   std::shared_ptr<PlanImpl> p = checked(handle);
@@ -272,12 +284,7 @@ flagfftResult exec_common(flagfftHandle handle, const void *in, void *out, flagf
   if (p->desc.type != expect) return FLAGFFT_INVALID_TYPE;
   if (has_direction && direction != FLAGFFT_FORWARD && direction != FLAGFFT_INVERSE) return FLAGFFT_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(p->mu);
-  Executable *ex = nullptr;
-  if (p->desc.real_fwd) ex = p->fwd.get();
-  else if (p->desc.real_inv) ex = p->inv.get();
-  else ex = direction == FLAGFFT_INVERSE ? p->inv.get() : p->fwd.get();
-  if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
-  if (in == out && ex->inplace_hazard) ex = ex == p->fwd.get() ? p->fwd_ip.get() : p->inv_ip.get();
+  Executable *ex = select_exec(*p, in, out, direction);
   if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
   int cur = p->device;
   if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);
@@ -450,8 +457,10 @@ extern "C" flagfftResult flagfftDestroy(flagfftHandle handle) {
     p = std::move(it->second);
     flagfft::g_handles.erase(it);
   }
+  // The handle shell stays allocated (a tombstone with a null impl): its
+  // address is never handed out again, so a stale handle cannot alias a plan
+  // created later and keeps returning FLAGFFT_INVALID_PLAN.
   handle->impl = nullptr;
-  delete handle;
   // the plan (workspaces, tables, graphs) goes with the last reference: here,
   // or at the end of a call that looked the handle up before the erase
   p.reset();
@@ -626,3 +635,4 @@ extern "C" flagfftResult flagfftxTunedDbPath(char *buf, size_t len) {
   buf[len - 1] = '\0';
   return FLAGFFT_SUCCESS;
 }
+

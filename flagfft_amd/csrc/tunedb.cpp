@@ -23,6 +23,18 @@
 #include <sstream>
 
 namespace flagfft {
+
+// Planner and kernel fingerprints: hashes of the planner sources and of the
+// device sources, computed by the build (flagfft_amd/build.py), so a row
+// measured with other kernels or another planner never applies.
+#ifndef FLAGFFT_PLANNER_HASH
+#error "FLAGFFT_PLANNER_HASH must be defined by the build"
+#endif
+#ifndef FLAGFFT_KERNEL_HASH
+#error "FLAGFFT_KERNEL_HASH must be defined by the build"
+#endif
+constexpr const char *kPlannerFingerprint = "gfx950-planner-" FLAGFFT_PLANNER_HASH;
+constexpr const char *kCodegenFingerprint = "gfx950-kernels-" FLAGFFT_KERNEL_HASH;
 namespace {
 
 struct sqlite3;

@@ -9,8 +9,6 @@ namespace flagfft {
 
 constexpr int64_t kTuneSchemaVersion = 3;
 constexpr const char *kTuneDeviceArch = "gfx950";
-constexpr const char *kPlannerFingerprint = "gfx950-planner-1";
-constexpr const char *kCodegenFingerprint = "gfx950-kernels-2";
 constexpr const char *kRuntimeFingerprint = "hip-runtime-1";
 
 // Lookup key of one transform kind (1D).

@@ -11,6 +11,7 @@ src/plan/c_api_internal.hpp:5-7).
   and buffers, give bit-identical results to the same calls made one after
   the other on one thread.
 """
+import ctypes
 import threading
 import time
 
@@ -98,3 +99,29 @@ def test_two_plans_from_two_threads(gpu):
         torch.cuda.synchronize(gpu)
         for p in plans:
             p.destroy()
+
+
+def test_destroyed_handle_is_never_reused(gpu):
+    """A destroyed handle keeps returning FLAGFFT_INVALID_PLAN after new plans
+    are created: its address is not handed out again."""
+    import torch
+
+    x = torch.zeros((2, 256), dtype=torch.complex64, device=gpu)
+    r, old = ff.plan1d(256, ff.Type.C2C, 2)
+    ff.check("flagfftPlan1d", r)
+    stale = old.value
+    ff.check("flagfftDestroy", ff.destroy(old))
+    fresh = []
+    try:
+        for _ in range(64):
+            r, h = ff.plan1d(256, ff.Type.C2C, 2)
+            ff.check("flagfftPlan1d", r)
+            fresh.append(h)
+        assert stale not in {h.value for h in fresh}
+        assert ff.exec_c2c(ctypes.c_void_p(stale), x.data_ptr(), x.data_ptr(), ff.FORWARD) == ff.Result.INVALID_PLAN
+        assert ff.destroy(ctypes.c_void_p(stale)) == ff.Result.INVALID_PLAN
+        assert ff.exec_c2c(fresh[-1], x.data_ptr(), x.data_ptr(), ff.FORWARD) == ff.Result.SUCCESS
+        torch.cuda.synchronize(gpu)
+    finally:
+        for h in fresh:
+            ff.destroy(h)

@@ -97,3 +97,75 @@ def test_split_precision_same_data_as_vector_path(gpu):
     assert errs[ff.COMPUTE_VECTOR] <= 2e-6 * math.log2(N)
     assert errs[ff.COMPUTE_FP16X3] <= BOUND[ff.COMPUTE_FP16X3]
     assert errs[ff.COMPUTE_BF16X3] <= BOUND[ff.COMPUTE_BF16X3]
+
+
+# Rows sampled from every part of a 65536-transform batch: the first wave
+# iteration (rows below 4 x 1024 groups), the second and later iterations of
+# each wave's loop (rows past 4096, read by the software-pipelined prefetch),
+# and the last transforms of the batch.
+FULL_BATCH = 65536
+SAMPLE_ROWS = np.r_[0:64, 4096:4160, 8192:8200, 32768:32832, 65472:65536]
+
+
+def _device_normal(dev, batch, seed):
+    """Complex N(0,1) rows (re and im iid N(0,1)) generated on the device."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    return torch.randn((batch, N), dtype=torch.complex64, device=dev, generator=g) * (2.0 ** 0.5)
+
+
+@pytest.mark.parametrize("direction", [ff.FORWARD, ff.INVERSE], ids=["fwd", "inv"])
+def test_full_configuration_1024x65536_all_paths(gpu, direction):
+    """N=1024 x 65536 (the precision-form configuration) on the same data for the
+    FP32 vector path, FP16 hi+lo and BF16 hi+lo: sampled rows against the FP64
+    oracle, every row of each split path against the vector path."""
+    import torch
+
+    xd = _device_normal(gpu, FULL_BATCH, 1024 + direction)
+    x = xd[SAMPLE_ROWS].cpu().numpy().astype(np.complex128)
+    ref = ora.fft(x, sign=direction)
+    outs, errs = {}, {}
+    for compute in (ff.COMPUTE_VECTOR, ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3):
+        yd = torch.empty_like(xd)
+        with ff.Plan.one_d_compute(N, ff.Type.C2C, FULL_BATCH, compute) as p:
+            p.set_stream(stream_of(gpu))
+            p.c2c(xd.data_ptr(), yd.data_ptr(), direction)
+            sync(gpu)
+            if compute != ff.COMPUTE_VECTOR:
+                assert "k_mfma1024" in p.description()
+        errs[compute] = ora.rel_l2(yd[SAMPLE_ROWS].cpu().numpy(), ref, N)[0]
+        outs[compute] = yd
+    bound = {ff.COMPUTE_VECTOR: 2e-6 * math.log2(N), **BOUND}
+    print(f"N=1024 x {FULL_BATCH} dir={direction} rel_l2 vs FP64 oracle (rows {len(SAMPLE_ROWS)}): "
+          + ", ".join(f"{NAMES.get(k, 'vector')}={v:.3e}" for k, v in errs.items()))
+    for compute, e in errs.items():
+        assert np.isfinite(e) and e <= bound[compute], (compute, e)
+    # every row: the split paths against the vector path (worst row), within the sum of both bounds
+    vec = outs[ff.COMPUTE_VECTOR]
+    vnorm = torch.linalg.vector_norm(vec, dim=1)
+    for compute in (ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3):
+        d = torch.linalg.vector_norm(outs[compute] - vec, dim=1) / vnorm
+        worst = float(d.max())
+        print(f"  {NAMES[compute]} worst row vs vector path: {worst:.3e} (row {int(d.argmax())})")
+        assert worst == worst and worst <= bound[compute] + bound[ff.COMPUTE_VECTOR]
+
+
+@pytest.mark.parametrize("compute", [ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3], ids=["fp16x3", "bf16x3"])
+@pytest.mark.parametrize("direction", [ff.FORWARD, ff.INVERSE], ids=["fwd", "inv"])
+def test_wave_loop_and_prefetch_with_few_groups(gpu, compute, direction):
+    """Two workgroups (8 waves) for 200 transforms: every wave runs its loop
+    25 times, each iteration on the transform its previous one prefetched."""
+    batch = 200
+    x = complex_input(ora.accuracy_seed(int(ff.Type.C2C), N, batch, 10), (batch, N), np.complex64, "normal")
+    ff.check("flagfftxSetTuning", ff.set_tuning(units=2))
+    try:
+        y, desc = _run(gpu, x, direction, compute)
+    finally:
+        ff.set_tuning()
+    assert "groups=2" in desc, desc
+    ref = ora.fft(x.astype(np.complex128), sign=direction)
+    l2, _ = ora.rel_l2(y, ref, N)
+    print(f"{NAMES[compute]} 2 groups x 4 waves, batch {batch}: rel_l2={l2:.3e}")
+    assert np.isfinite(l2) and l2 <= BOUND[compute]

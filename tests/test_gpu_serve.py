@@ -152,3 +152,98 @@ def test_request_past_2_31_elements_is_refused(gpu):
             srv.submit(T.C2C, 4096, (1 << 19) + 1, x.data_ptr(), ff.FORWARD)
         assert e.value.code == ff.Result.INVALID_SIZE
         assert srv.stats()["completed"] == 0
+
+
+# ---- the serving configuration's request stream ------------------------------
+def _trace(nreq: int):
+    """Requests of the serving configuration (length log-uniform in [64, 2^20],
+    20% primes, batch uniform in [1, 512], 70% C2C f32 / 20% R2C f32 / 10% Z2Z
+    f64), the same generator the benchmark's serving leg uses, plus one
+    2^20 x 256 C2C request and one prime-length request so that both kinds
+    are always present."""
+    from bench import _is_prime, serving_trace
+
+    tm = {"C2C": T.C2C, "R2C": T.R2C, "Z2Z": T.Z2Z}
+    reqs = [(tm[t], n, b) for t, n, b in serving_trace(nreq, 2024)]
+    reqs.insert(nreq // 3, (T.C2C, 1 << 20, 256))
+    reqs.insert(2 * nreq // 3, (T.C2C, 65537, 97))
+    assert any(_is_prime(n) for _, n, _ in reqs)
+    return reqs
+
+
+def _sample_rows(b: int):
+    return sorted({0, b // 2, b - 1})
+
+
+def test_serving_trace_poisson_alone_and_in_the_mix(gpu):
+    """The serving configuration's stream at reduced length, with Poisson
+    arrivals: every request's output is bit-identical served alone and in the
+    mix, and its first, middle and last transforms match the FP64 oracle."""
+    import time
+
+    import torch
+
+    from .helpers import hip_runtime
+
+    reqs = _trace(30)
+    # one input buffer per type, sized for its largest request; every request reads its prefix
+    count = {T.C2C: 2, T.R2C: 1, T.Z2Z: 2}
+    dtype = {T.C2C: torch.float32, T.R2C: torch.float32, T.Z2Z: torch.float64}
+    need = {t: max([count[t] * n * b for tt, n, b in reqs if tt == t] or [2]) for t in count}
+    g = torch.Generator(device=gpu)
+    g.manual_seed(77)
+    inputs = {t: torch.randn(need[t], dtype=dtype[t], device=gpu, generator=g) for t in count}
+    host_in = {t: v.cpu().numpy() for t, v in inputs.items()}
+    sync(gpu)
+
+    def out_bytes(t, n, b):
+        shape, dt = _out_spec(t, n, b)
+        return int(np.prod(shape)) * np.dtype(dt).itemsize
+
+    def keep(ptr, t, n, b):
+        dst = torch.empty(out_bytes(t, n, b), dtype=torch.uint8, device=gpu)
+        assert hip_runtime().hipMemcpy(dst.data_ptr(), ptr, dst.numel(), 3) == 0  # device to device
+        return dst
+
+    with ff.Server(32 << 30, max_inflight=3) as srv:
+        alone = []
+        for t, n, b in reqs:
+            tk = srv.submit(t, n, b, inputs[t].data_ptr(), ff.FORWARD)
+            ptr, _ = srv.wait(tk)
+            alone.append(keep(ptr, t, n, b))
+            srv.release(tk)
+        srv.reset_stats()
+        rng = np.random.default_rng(5)
+        gaps = rng.exponential(0.4e-3, len(reqs))  # Poisson arrivals, 2500 requests/s offered
+        tickets = []
+        t_next = time.perf_counter()
+        for (t, n, b), gap in zip(reqs, gaps):
+            t_next += gap
+            while time.perf_counter() < t_next:
+                pass
+            tickets.append(srv.submit(t, n, b, inputs[t].data_ptr(), ff.FORWARD))
+        mixed = []
+        for tk, (t, n, b) in zip(tickets, reqs):
+            ptr, lat = srv.wait(tk)
+            assert lat >= 0
+            mixed.append(keep(ptr, t, n, b))
+            srv.release(tk)
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    print(f"serving trace: {len(reqs)} requests, {st['transforms']} transforms, queue launches "
+          f"{st['queue_launches']}, queued requests {st['queued_requests']}, p50 {st['p50_ms']:.3f} ms, "
+          f"p99 {st['p99_ms']:.3f} ms")
+    for (t, n, b), a, m in zip(reqs, alone, mixed):
+        assert torch.equal(a, m), f"{t.name} n={n} batch={b}: result differs between alone and in the mix"
+        rows = _sample_rows(b)
+        shape, dt = _out_spec(t, n, b)
+        y = m.view(b, -1)[rows].cpu().numpy().view(dt).reshape(len(rows), shape[-1])
+        flat = host_in[t]
+        if t == T.C2C:
+            x = flat[: 2 * n * b].view(np.complex64).reshape(b, n)[rows]
+        elif t == T.Z2Z:
+            x = flat[: 2 * n * b].view(np.complex128).reshape(b, n)[rows]
+        else:
+            x = flat[: n * b].reshape(b, n)[rows]
+        l2, _ = ora.rel_l2(y, _oracle(t, n, x, ff.FORWARD), y.shape[-1])
+        assert l2 <= _bound(t, n), f"{t.name} n={n} batch={b} rel_l2={l2:.3e}"

@@ -174,6 +174,71 @@ def test_c2r_drops_dc_and_nyquist_imaginary(n):
     assert rel(ora.irfft(ora.rfft(x), n), n * x) < 1e-14
 
 
+def _hermitian_full(X, n):
+    """Full n-point spectrum of a Hermitian half spectrum (Im of DC / Nyquist dropped)."""
+    Xh = X.copy()
+    Xh[..., 0] = Xh[..., 0].real
+    if n % 2 == 0:
+        Xh[..., -1] = Xh[..., -1].real
+    return np.concatenate([Xh, np.conj(Xh[..., 1:(n + 1) // 2][..., ::-1])], axis=-1)
+
+
+@pytest.mark.parametrize("n", [2, 3, 15, 16, 67, 1000, 4096, 6000, 10007])
+def test_fast_real_f32_against_the_long_double_dft(n):
+    """FP32 R2C / C2R (the FP32 real CPU-baseline path) against the long-double
+    DFT of the same FP32 data, at the FP32 error level."""
+    bound = 2e-7 * max(1.0, np.log2(n))
+    x = ora.signed_unit(11 * n, 3 * n).reshape(3, n).astype(np.float32)
+    X = ora.rfft_f32(x)
+    assert X.dtype == np.complex64 and X.shape == (3, n // 2 + 1)
+    assert rel(X, ora.dft(x.astype(np.complex128))[:, : n // 2 + 1]) < bound
+    H = ora.complex_signed_unit(13 * n, (3, n // 2 + 1)).astype(np.complex64)
+    y = ora.irfft_f32(H, n)
+    assert y.dtype == np.float32 and y.shape == (3, n)
+    assert rel(y, ora.dft(_hermitian_full(H.astype(np.complex128), n), sign=1).real) < bound
+    assert rel(ora.irfft_f32(X, n), n * x.astype(np.float64)) < 2 * bound
+
+
+def test_fast_real_f32_matches_golden():
+    """FP32 R2C / C2R against the numpy (pocketfft) fixtures of the FP64 data,
+    within the FP32 rounding of the input and the transform."""
+    for key in GOLDEN.files:
+        if key.endswith("_seed") or not key.startswith(("r2c_", "c2r_")):
+            continue
+        n = int(key.split("_")[-1])
+        seed = int(GOLDEN[key + "_seed"])
+        if key.startswith("r2c_"):
+            got = ora.rfft_f32(ora.signed_unit(seed, n).astype(np.float32))
+        else:
+            got = ora.irfft_f32(ora.complex_signed_unit(seed, (n // 2 + 1,)).astype(np.complex64), n)
+        assert rel(got, GOLDEN[key]) < 3e-7 * max(1.0, np.log2(n)), key
+
+
+def test_fast_real_f32_planted_mistakes_are_caught():
+    """A conjugated spectrum, a dropped input sample, a Nyquist bin counted
+    twice and a reversed output each fail the checks above; C2R ignores a large
+    imaginary Nyquist part."""
+    n = 6000
+    x = ora.signed_unit(99, n).reshape(1, n).astype(np.float32)
+    ref = ora.dft(x.astype(np.complex128))[:, : n // 2 + 1]
+    X = ora.rfft_f32(x)
+    assert rel(X, ref) < 1e-6
+    assert rel(np.conj(X), ref) > 0.5
+    xd = x.copy()
+    xd[0, 1234] = 0
+    assert rel(ora.rfft_f32(xd), ref) > 1e-3
+    H = ora.complex_signed_unit(7, (1, n // 2 + 1))
+    H[0, -1] += 1000j  # a large imaginary Nyquist part that C2R must ignore
+    want = ora.dft(_hermitian_full(H, n), sign=1).real
+    got = ora.irfft_f32(H.astype(np.complex64), n)
+    assert rel(got, want) < 1e-6
+    # the Nyquist bin mirrored as if it had a partner (counted twice)
+    twice = _hermitian_full(H, n)
+    twice[:, n // 2] *= 2
+    assert rel(ora.dft(twice, sign=1).real, want) > 1e-3
+    assert rel(got[:, ::-1], want) > 0.5
+
+
 def test_c2r_zero_is_exact_zero():
     """ctest/test_exec_c2r_ct_s.cpp:76-96."""
     out = ora.irfft(np.zeros((1, 129), complex), 256)
