@@ -49,10 +49,11 @@ def _factor(n: int):
 
 
 def _radices(n: int):
-    """First-fit-decreasing packing of the prime factors into radices <= 16 (the planner's default)."""
+    """First-fit-decreasing packing of the prime factors into radices <= 16,
+    with 17 and 19 as radices of their own (the planner's default)."""
     bins = []
     for p in sorted(_factor(n), reverse=True):
-        if p > 16:
+        if p > 19:
             return None
         for i, b in enumerate(bins):
             if b * p <= 16:

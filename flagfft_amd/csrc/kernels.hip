@@ -147,7 +147,7 @@ __device__ __noinline__ void stage_last(LdsC<T> *buf, int R, int q, int twstep, 
   }
 }
 
-// Single-stage transform (R <= 16): HBM -> registers -> HBM.
+// Single-stage transform (one radix: R <= 17 or R == 19): HBM -> registers -> HBM.
 template <typename T, int RR, int E>
 __device__ __noinline__ void stage_only(FLAGFFT_SRC_PARAMS, FLAGFFT_DST_PARAMS, int lt) {
   if (lt != 0 || !s_active) return;
@@ -159,7 +159,7 @@ __device__ __noinline__ void stage_only(FLAGFFT_SRC_PARAMS, FLAGFFT_DST_PARAMS, 
   for (int j = 0; j < RR; ++j) dst_elem<T>(d_out, d_base, d_es, d_mode, d_nlog, d_chirp, d_bhat, d_conj, d_scale, j, v[j]);
 }
 
-#define FLAGFFT_RADICES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#define FLAGFFT_RADICES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(19)
 
 // Forward DFT of length R held in LDS (natural order in, natural order out).
 // Every thread of the block must call this (it contains block barriers).
@@ -175,7 +175,7 @@ __device__ __forceinline__ void lds_fft(LdsC<T> *buf, int R, int nst, const int3
   case RR: lds_stage<T, RR, E>(buf, R, q, fdq[s], twstep[s], lt, tpu, tw); break;
       FLAGFFT_RADICES(FLAGFFT_MID)
 #undef FLAGFFT_MID
-      default: break;  // the planner emits radices 2..16 only
+      default: break;  // the planner emits radices 2..17 and 19 only
     }
     q *= r;
   }

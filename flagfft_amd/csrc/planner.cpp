@@ -79,14 +79,24 @@ bool smooth13(int64_t n) {
   return n == 1;
 }
 
-// First-fit-decreasing packing of the prime factors into radices <= 16.
+// Lengths whose prime factors all have a radix codelet: 2, 3, 5, 7, 11, 13,
+// 17, 19 (the reference's supported radices, src/utils/include/flagfft/base.hpp:47).
+bool smooth19(int64_t n) {
+  if (n < 1) return false;
+  for (int64_t p : {2, 3, 5, 7, 11, 13, 17, 19})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// First-fit-decreasing packing of the prime factors into radices <= 16;
+// the primes 17 and 19 are radices of their own.
 bool radix_factor(int64_t n, std::vector<int> &out) {
   out.clear();
   if (n < 1) return false;
   if (n == 1) return true;
   std::vector<int64_t> f = prime_factors(n);
   for (int64_t p : f)
-    if (p > 13) return false;
+    if (p > 19 || p == 18) return false;
   std::sort(f.begin(), f.end(), std::greater<int64_t>());
   std::vector<int> bins;
   for (int64_t p : f) {
@@ -224,7 +234,7 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       int64_t prod = 1;
       for (int r : o.forced_radices) prod *= r;
       bool valid = prod == n;
-      for (int r : o.forced_radices) valid = valid && r >= 2 && r <= 16;
+      for (int r : o.forced_radices) valid = valid && r >= 2 && (r <= 17 || r == 19);
       if (valid) rad = o.forced_radices;
     }
     c.radices = rad;
@@ -240,7 +250,7 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       return c;
     }
   }
-  if (smooth13(n) && multipass_core(n, p, o, c)) return c;
+  if (smooth19(n) && multipass_core(n, p, o, c)) return c;
   // Bluestein over a 13-smooth convolution length M >= 2n-1.  A length that
   // fits one LDS pass is taken as soon as it is found (the whole convolution
   // is one launch).  Otherwise every M up to twice the minimum is weighed by

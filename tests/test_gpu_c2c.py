@@ -109,3 +109,44 @@ def test_large_four_step(gpu):
         x = complex_input(0xB16 + n, (batch, n), np.complex64, kind="normal")
         y = run_c2c(gpu, x, ff.FORWARD, n, batch)
         check_accuracy(y, ora.fft(x.astype(np.complex128)), ff.Type.C2C, n, row_len=n, what="large")
+
+
+# Lengths whose factors include the primes 17 and 19, which have radix
+# codelets of their own (the reference's kSupportedRadices,
+# src/utils/include/flagfft/base.hpp:47): single-pass leaves, multi-pass
+# chains and a Rader convolution of length 102 = 17 * 6, never Bluestein.
+RADIX_17_19 = [17, 19, 289, 361, 4352, 5168, 6859, 9728, 83521, 103, 2 * 17 * 19 * 1024]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", RADIX_17_19)
+def test_radix_17_and_19_lengths(gpu, n, dtype):
+    t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
+    r, text = ff.describe_host(1, [n], t, 3)
+    assert r == ff.Result.SUCCESS and "Bluestein(" not in text, text
+    for direction in (ff.FORWARD, ff.INVERSE):
+        _case(gpu, n, 3, dtype, direction)
+
+
+@pytest.mark.parametrize("n", [2 * 17, 2 * 361, 4352, 2 * 6859])
+def test_radix_17_and_19_real_roundtrip(gpu, n):
+    """R2C against the oracle, then C2R back to n x the input, in FP32 and FP64."""
+    from .helpers import dev_tensor, host, real_input, stream_of, sync
+
+    for dbl in (False, True):
+        rdt = np.float64 if dbl else np.float32
+        fwd, inv = (ff.Type.D2Z, ff.Type.Z2D) if dbl else (ff.Type.R2C, ff.Type.C2R)
+        x = real_input(ora.accuracy_seed(int(fwd), n, 3, 0), (3, n), rdt)
+        xd = dev_tensor(x, gpu)
+        import torch
+
+        yd = torch.empty((3, n // 2 + 1), dtype=torch.complex128 if dbl else torch.complex64, device=gpu)
+        zd = torch.empty_like(xd)
+        with ff.Plan.one_d(n, fwd, 3) as pf, ff.Plan.one_d(n, inv, 3) as pi:
+            pf.set_stream(stream_of(gpu))
+            pi.set_stream(stream_of(gpu))
+            (pf.d2z if dbl else pf.r2c)(xd.data_ptr(), yd.data_ptr())
+            (pi.z2d if dbl else pi.c2r)(yd.data_ptr(), zd.data_ptr())
+            sync(gpu)
+        check_accuracy(host(yd), ora.rfft(x.astype(np.float64)), fwd, n, row_len=n // 2 + 1, what="r2c")
+        check_accuracy(host(zd), n * x.astype(np.float64), inv, n, row_len=n, factor=2.0, what="roundtrip")
