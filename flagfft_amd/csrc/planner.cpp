@@ -96,7 +96,7 @@ bool radix_factor(int64_t n, std::vector<int> &out) {
   if (n == 1) return true;
   std::vector<int64_t> f = prime_factors(n);
   for (int64_t p : f)
-    if (p > 19 || p == 18) return false;
+    if (p > 19) return false;
   std::sort(f.begin(), f.end(), std::greater<int64_t>());
   std::vector<int> bins;
   for (int64_t p : f) {

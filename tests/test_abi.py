@@ -219,7 +219,7 @@ def _core_checks(n, text):
         assert math.prod(parts) == nn and all(2 <= v <= 4096 for v in parts)
     for m in re.finditer(r"(?<!super_)radices=\[([\d,]*)\]", text):
         rad = [int(v) for v in m.group(1).split(",") if v]
-        assert all(2 <= v <= 16 for v in rad)
+        assert all(2 <= v <= 17 or v == 19 for v in rad)
     if leaf and "Rader" not in text and "Bluestein" not in text and not multi:
         rad = [int(v) for v in leaf.group(2).split(",") if v]
         assert math.prod(rad) == int(leaf.group(1))

@@ -15,11 +15,11 @@ def _describe(n, t, batch):
 
 
 def test_radix_packing_matches_the_planner():
-    for n in (16, 256, 1000, 3000, 4096, 10935, 6561):
+    for n in (16, 256, 1000, 3000, 4096, 10935, 6561, 17, 361, 4352, 5168):
         rad = at._radices(n)
-        assert int(np.prod(rad)) == n and all(2 <= r <= 16 for r in rad)
+        assert int(np.prod(rad)) == n and all(2 <= r <= 17 or r == 19 for r in rad)
         assert f"radices=[{','.join(map(str, rad))}]" in _describe(n, T.C2C, 8)
-    assert at._radices(17) is None
+    assert at._radices(23) is None and at._radices(17 * 23) is None
 
 
 @pytest.mark.parametrize("n,k", [(1 << 20, 2), (1 << 24, 2), (10935, 2), (20480, 3)])

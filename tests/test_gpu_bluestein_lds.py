@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 F32 = [(4099, "conv_length=16384, fused=1, lds_resident=1)"), (9001, "conv_length=20160, fused=1, lds_resident=1)"),
        (12011, "conv_length=32768, fused=1, lds_resident=1, halves=1"),
        (20011, "conv_length=40320, fused=1, lds_resident=1, halves=1")]
-F64 = [(2053, "conv_length=8192, fused=1, lds_resident=1)"), (4099, "conv_length=10080, fused=1, lds_resident=1)"),
+F64 = [(2063, "conv_length=8192, fused=1, lds_resident=1)"), (4099, "conv_length=10080, fused=1, lds_resident=1)"),
        (7919, "conv_length=16384, fused=1, lds_resident=1, halves=1"),
        (10007, "conv_length=20160, fused=1, lds_resident=1, halves=1")]
 
