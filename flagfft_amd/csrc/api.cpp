@@ -439,6 +439,86 @@ extern "C" flagfftResult flagfftExecZ2D(flagfftHandle plan, flagfftDoubleComplex
   return exec_common(plan, idata, odata, FLAGFFT_Z2D, FLAGFFT_INVERSE, false);
 }
 
+// ---- allocation-free execution ----------------------------------------------
+//
+// Per-device workspace of flagfftxExecC2CNoAlloc: a fixed array in the code
+// object, so the call needs no hipMalloc.  Every device that loads the
+// library holds one copy until the process ends.  In host code `scratch`
+// names a host-side stand-in, never the device memory: kernels get the
+// address from hipGetSymbolAddress (per device, kept in g_scratch_ptr).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace flagfft {
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device ordinal -> address of that device's `scratch`
+
+// The device address of `scratch` on `device` (current when called);
+// looked up once per device.
+hipError_t scratch_on(int device, void **ptr) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  }
+  g_scratch_ptr[device] = p;
+  *ptr = p;
+  return hipSuccess;
+}
+
+}  // namespace
+}  // namespace flagfft
+
+extern "C" flagfftResult flagfftxExecC2CNoAlloc(flagfftHandle plan, flagfftComplex *idata, flagfftComplex *odata,
+                                                int direction) {
+  std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(plan);
+  if (p == nullptr) return FLAGFFT_INVALID_PLAN;
+  if (idata == nullptr || odata == nullptr) return FLAGFFT_INVALID_VALUE;
+  if (p->desc.type != FLAGFFT_C2C) return FLAGFFT_INVALID_TYPE;
+  if (direction != FLAGFFT_FORWARD && direction != FLAGFFT_INVERSE) return FLAGFFT_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(p->mu);
+  flagfft::Executable *ex = flagfft::select_exec(*p, idata, odata, direction);
+  if (ex == nullptr) return FLAGFFT_INTERNAL_ERROR;
+  // the plan's workspaces, each at a 256-byte aligned offset of the scratch array
+  constexpr size_t kMaxWs = 8;
+  size_t offset[kMaxWs] = {};
+  size_t need = 0;
+  if (ex->ws_bytes.size() > kMaxWs) return FLAGFFT_INVALID_SIZE;
+  for (size_t i = 0; i < ex->ws_bytes.size(); ++i) {
+    offset[i] = need;
+    need += (std::max<size_t>(ex->ws_bytes[i], 256) + 255) / 256 * 256;
+  }
+  if (need > kNoAllocScratchBytes) {
+    std::fprintf(stderr, "[flagfft] flagfftxExecC2CNoAlloc: the plan needs %zu bytes of workspace, above the %zu-byte "
+                 "scratch array\n", need, kNoAllocScratchBytes);
+    return FLAGFFT_INVALID_SIZE;
+  }
+  int cur = p->device;
+  if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);
+  void *base = nullptr;
+  hipError_t e = flagfft::scratch_on(p->device, &base);
+  if (e == hipSuccess) {
+    void *wsp[kMaxWs] = {};
+    for (size_t i = 0; i < ex->ws_bytes.size(); ++i) wsp[i] = static_cast<unsigned char *>(base) + offset[i];
+    e = ex->run_batch(idata, odata, wsp, ex->built_batch, p->stream);
+  }
+  if (cur != p->device) (void)hipSetDevice(cur);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "[flagfft] exec failed: %s\n", hipGetErrorString(e));
+    return FLAGFFT_EXEC_FAILED;
+  }
+  return FLAGFFT_SUCCESS;
+}
+
 extern "C" flagfftResult flagfftSetStream(flagfftHandle handle, flagfftStream_t stream) {
   std::shared_ptr<flagfft::PlanImpl> p = flagfft::checked(handle);
   if (p == nullptr) return FLAGFFT_INVALID_PLAN;

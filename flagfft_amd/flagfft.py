@@ -55,7 +55,8 @@ EXT_SYMBOLS = ["flagfftxVersion", "flagfftxPlanDescribe", "flagfftxPlanStats", "
                "flagfftxSetProfiling", "flagfftxGetProfile", "flagfftxStepLabel", "flagfftxPlan1dCompute", "flagfftxPlan3d", "flagfftxPlanMany",
                "flagfftxSetGraphMode", "flagfftxServerCreate", "flagfftxServerPrepare", "flagfftxServerSubmit",
                "flagfftxServerPoll", "flagfftxServerWait", "flagfftxServerRelease", "flagfftxServerGetStats",
-               "flagfftxServerResetStats", "flagfftxServerDestroy", "flagfftxTunedStore", "flagfftxTunedDbPath"]
+               "flagfftxServerResetStats", "flagfftxServerDestroy", "flagfftxTunedStore", "flagfftxTunedDbPath",
+               "flagfftxExecC2CNoAlloc"]
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
 
@@ -115,6 +116,7 @@ def lib():
             "flagfftPlan3d": [_HP, _int, _int, _int, _int],
             "flagfftPlanMany": [_HP, _int, _IP, _IP, _int, _int, _IP, _int, _int, _int, _int],
             "flagfftExecC2C": [_H, _vp, _vp, _int],
+            "flagfftxExecC2CNoAlloc": [_H, _vp, _vp, _int],
             "flagfftExecZ2Z": [_H, _vp, _vp, _int],
             "flagfftExecR2C": [_H, _vp, _vp],
             "flagfftExecD2Z": [_H, _vp, _vp],
@@ -214,6 +216,11 @@ def plan_many_ext(rank, n, inembed, istride, idist, onembed, ostride, odist, typ
 
 def exec_c2c(h, idata: int, odata: int, direction: int) -> int:
     return lib().flagfftExecC2C(h, idata, odata, direction)
+
+
+def exec_c2c_noalloc(h, idata: int, odata: int, direction: int) -> int:
+    """flagfftxExecC2CNoAlloc: flagfftExecC2C with the workspaces in the library's fixed per-device array."""
+    return lib().flagfftxExecC2CNoAlloc(h, idata, odata, direction)
 
 
 def exec_z2z(h, idata: int, odata: int, direction: int) -> int:

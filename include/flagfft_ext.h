@@ -85,6 +85,18 @@ flagfftResult flagfftxTunedDbPath(char *buf, size_t len);
  * default mode.  While profiling is enabled Exec launches directly. */
 flagfftResult flagfftxSetGraphMode(flagfftHandle plan, int enable);
 
+/* flagfftExecC2C without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The plan's workspaces
+ * are taken from one fixed 64 MiB array per device, held from the library's
+ * first use on that device until the process ends, instead of the plan's own;
+ * a plan needing more (e.g. a multi-pass C2C past 8M elements per exec)
+ * returns FLAGFFT_INVALID_SIZE.  Results are bit-identical to
+ * flagfftExecC2C.  All calls on one device share that array: calls that
+ * overlap on one device (other streams, other threads) race and must be
+ * serialised by the caller. */
+flagfftResult flagfftxExecC2CNoAlloc(flagfftHandle plan, flagfftComplex *idata, flagfftComplex *odata, int direction);
+
 /* Arithmetic of a plan.  VECTOR is the FP32/FP64 path of flagfftPlan1d.  The
  * split-precision modes run the two radix-32 stages of a length-1024 C2C
  * transform as dense complex matrix products on the matrix cores: every FP32
