@@ -19,7 +19,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libflagfft.so")
 ARCH = os.environ.get("FLAGFFT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "leaf.hip", "stage.hip", "bluestein_lds.hip", "leaf_split.hip", "mfma.hip", "planner.cpp", "exec.cpp", "api.cpp", "serve.cpp", "tunedb.cpp"]
+SOURCES = ["kernels.hip", "leaf.hip", "stage.hip", "colpass.hip", "bluestein_lds.hip", "leaf_split.hip", "mfma.hip", "planner.cpp", "exec.cpp", "api.cpp", "serve.cpp", "tunedb.cpp"]
 HEADERS = ["fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp", "planner.hpp", "exec.hpp", "tunedb.hpp"]
 
 COMMON_FLAGS = [

@@ -102,7 +102,8 @@ size_t Step::pack_args(void *src, void *dst, unsigned char *buf, void *aux) cons
   switch (kind) {
     case Pass: return put(pa);
     case Leaf: return put(la);
-    case Stage: return put(sa);
+    case Stage:
+    case Col: return put(sa);
     case Mfma: return put(ma);
     case BluesteinLds: {
       BluesteinLdsArgs b = ba;
@@ -132,6 +133,15 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       s.pa.nblocks = static_cast<int32_t>(blocks(s.pa.units, pa.C));
       s.shape.grid = dim3(static_cast<unsigned>(s.pa.nblocks));
       break;
+    case Col: {
+      int per = 1, thr = 0, ebytes = 0;
+      size_t lds = 0;
+      col_kernel_shape(leaf_id, per, thr, lds, ebytes);
+      s.sa.units = scale(sa.units);
+      s.sa.nblocks = static_cast<int32_t>(blocks(s.sa.units, per));
+      s.shape.grid = dim3(static_cast<unsigned>(s.sa.nblocks));
+      break;
+    }
     case Stage: {
       int per = 1, ept = 0;  // units per workgroup of the specialised kernel
       stage_kernel_shape(leaf_id, dbl, per, ept);
@@ -198,6 +208,7 @@ void Executable::resolve_kernels() {
       case Step::Pass: s.func = pass_kernel(s.dbl, s.ept); break;
       case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
+      case Step::Col: s.func = col_kernel(s.leaf_id); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
@@ -732,7 +743,12 @@ Builder::TwoLevel Builder::two_level(int64_t n) {
 bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst,
                         const SeqSet &s, int64_t n, int pass, int npasses) {
   const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
-  if (sid < 0 || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
+  // register-resident two-stage pass (plain complex chain passes)
+  const int cid = (mode == 0 && opt_.stage_form != 2 && !opt_.generic_kernels && a.src_mode == kSrcComplex &&
+                   a.dst_mode == kDstComplex)
+                      ? col_kernel_id(a.R, dbl(), opt_.forced_units)
+                      : -1;
+  if ((sid < 0 && cid < 0) || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
   // contiguous sequences, or the columns of contiguous rows of ncol elements
   const bool rows = !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1;
   const bool cols = s.col_fast && s.ncol > 1 && a.gi.es == s.ncol && a.gi.cs == 1 && a.go.es == s.ncol &&
@@ -742,6 +758,55 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   const int R = a.R;
   const int64_t tcnt = a.tcnt * ncol, p = a.p * ncol;
   if (tcnt >= (int64_t{1} << 31)) return false;
+  if (cid >= 0) {
+    int cunits = 0, cthreads = 0, cebytes = 0;
+    size_t clds = 0;
+    col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
+    Step cst;
+    cst.kind = Step::Col;
+    cst.dbl = dbl();
+    cst.src = src;
+    cst.dst = dst;
+    cst.leaf_id = cid;
+    StageArgs &g = cst.sa;
+    g.bs_in = a.gi.bs;
+    g.bs_out = a.go.bs;
+    g.tcnt = static_cast<int32_t>(tcnt);
+    g.p = static_cast<int32_t>(p);
+    g.twp_mult = a.twp_mult;
+    g.tw_bits = a.tw_bits;
+    g.fd_tcnt = make_fastdiv(static_cast<uint32_t>(tcnt));
+    g.fd_p = make_fastdiv(static_cast<uint32_t>(p));
+    g.ncol = static_cast<int32_t>(ncol);
+    g.fd_ncol = make_fastdiv(static_cast<uint32_t>(ncol));
+    g.tw_small = a.tw_small;
+    g.tw_lo = a.tw_lo;
+    g.tw_hi = a.tw_hi;
+    g.conj_in = a.conj_in;
+    g.conj_out = a.conj_out;
+    g.scale = a.scale;
+    g.units = a.units;
+    g.block_store = (rows && p == 1 && a.go.bs == n) ? 1 : 0;
+    g.src_mode = a.src_mode;
+    g.dst_mode = a.dst_mode;
+    g.nlog = a.nlog;
+    const int64_t nblocks = (a.units + cunits - 1) / cunits;
+    g.nblocks = static_cast<int32_t>(nblocks);
+    g.xcd_remap = opt_.xcd_remap;
+    cst.ept = 0;
+    cst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cthreads)), clds};
+    cst.bytes_moved =
+        s.nbatch * ncol * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+    std::ostringstream lab;
+    lab << "k_col<" << (dbl() ? "fp64" : "fp32") << "," << cunits << "> pass " << pass << "/" << npasses
+        << " R=" << R << " p=" << p << " tcnt=" << tcnt << (g.block_store ? " block-store" : "")
+        << (cols ? " columns=" : "");
+    if (cols) lab << ncol;
+    cst.label = lab.str();
+    ex_.steps.push_back(cst);
+    return true;
+  }
+  if (sid < 0) return false;
   int cu = 0, ept = 0;
   stage_kernel_shape(sid, dbl(), cu, ept);
   const int tpu = (R + ept - 1) / ept;

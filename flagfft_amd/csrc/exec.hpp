@@ -52,7 +52,8 @@ struct Step {
     C2REven,
     RealPost,
     RealPre,
-    Mfma
+    Mfma,
+    Col
   } kind = Pass;
   bool dbl = false;
   int ept = 16;

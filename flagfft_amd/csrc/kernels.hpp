@@ -45,6 +45,13 @@ int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units);
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept);
 const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 Bluestein convolution
 
+// Register-resident two-stage super-stages (colpass.hip): the instance for
+// super-radix r with `units` per workgroup (0: the default one), -1 if none;
+// its units, threads, dynamic LDS bytes and element bytes; the kernel.
+int col_kernel_id(int64_t r, bool dbl, int units);
+void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes);
+const void *col_kernel(int id);
+
 // Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
 // QueueArgs argument).
 const void *queue_kernel();

@@ -173,7 +173,50 @@ int column_segment_bytes(const std::vector<int> &rad, Prec p) {
   return 0;
 }
 
-bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c) {
+// Relative time of one HBM sweep by a pass of super-radix r: the inverse of
+// its measured rate on MI355X (TB/s of read + write bytes).  Register-resident
+// two-stage passes run at 3.8 (first pass, from uncached input) to 4.6;
+// LDS super-stages at 2.2 to 3.5 by the row-segment bytes their tile reads;
+// the run-time-radix pass at about 1.1.
+double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o) {
+  std::vector<int> rad;
+  if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0) return first ? 1.0 / 3.8 : 1.0 / 4.6;
+  if (radix_factor(r, rad)) {
+    const int seg = column_segment_bytes(rad, p);
+    if (seg >= 128) return 1.0 / 3.5;
+    if (seg >= 64) return 1.0 / 3.0;
+    if (seg > 0) return 1.0 / 2.2;
+  }
+  return 1.0 / 1.1;
+}
+
+// The split of n into 2-4 passes with the least total pass time; near ties
+// (2 %) go to fewer passes, then to the larger smallest factor.
+bool cheapest_split(int64_t n, Prec p, const PlanOptions &o, std::vector<int64_t> &best) {
+  const int64_t cap = pass_max(o, p);
+  double best_t = 0.0;
+  bool have = false;
+  for (int k = 2; k <= 4; ++k) {
+    std::vector<std::vector<int64_t>> all;
+    std::vector<int64_t> cur;
+    enum_splits(n, k, cap, cap, cur, all);
+    for (const auto &v : all) {
+      double tt = 0.0;
+      for (size_t i = 0; i < v.size(); ++i) tt += pass_time(v[i], p, i == 0, o);
+      const bool better = !have || tt < best_t * 0.98 ||
+                          (tt <= best_t * 1.02 && v.size() == best.size() &&
+                           *std::min_element(v.begin(), v.end()) > *std::min_element(best.begin(), best.end()));
+      if (better) {
+        best = v;
+        best_t = tt;
+        have = true;
+      }
+    }
+  }
+  return have;
+}
+
+bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c, bool by_cost = false) {
   std::vector<int64_t> passes;
   const int64_t cap = pass_max(o, p);
   bool ok = false;
@@ -190,6 +233,7 @@ bool multipass_core(int64_t n, Prec p, const PlanOptions &o, Core &c) {
       ok = true;
     }
   }
+  if (!ok && by_cost) ok = cheapest_split(n, p, o, passes);
   for (int k = 2; !ok && k <= 4; ++k) ok = split_passes(n, k, cap, passes);
   if (!ok) return false;
   c.kind = CoreKind::MultiPass;
@@ -250,7 +294,7 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       return c;
     }
   }
-  if (smooth19(n) && multipass_core(n, p, o, c)) return c;
+  if (smooth19(n) && multipass_core(n, p, o, c, true)) return c;
   // Bluestein over a 13-smooth convolution length M >= 2n-1.  A length that
   // fits one LDS pass is taken as soon as it is found (the whole convolution
   // is one launch).  Otherwise every M up to twice the minimum is weighed by

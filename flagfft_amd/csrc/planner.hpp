@@ -34,6 +34,7 @@ struct PlanOptions {
   std::vector<int64_t> forced_passes;  // explicit super-radix split (tuned plans)
   std::vector<int> forced_radices;     // explicit single-pass radix order (tuned plans)
   int forced_units = 0;                // explicit units per workgroup (tuned plans)
+  int stage_form = 0;                  // multi-pass chains: 0 automatic, 1 register-resident passes, 2 LDS super-stages
   int max_threads = 512;               // workgroup size limit of the launches
   bool generic_kernels = false;        // only the run-time-radix kernels (persistent-queue jobs)
   bool in_place = false;               // input and output are one buffer: no sequence split over workgroups
@@ -99,6 +100,8 @@ inline int lds_pad(int e) { return e + (e >> 5); }   // must match the kernels' 
 int stage_kernel_id(const std::vector<int> &radices);
 // Same for the specialised single-pass kernels.
 int leaf_kernel_id(const std::vector<int> &radices);
+// Register-resident two-stage pass instance for super-radix r (colpass.hip), -1 if none.
+int col_kernel_id(int64_t r, bool dbl, int units);
 // True when every super-radix of the split has a specialised kernel.
 bool passes_specialised(const std::vector<int64_t> &passes);
 

@@ -61,6 +61,9 @@ typedef struct flagfftxTuning_t {
   int passes[4];
   int nradices;       /* explicit single-pass radix order */
   int radices[16];
+  int stage_form;     /* kernel of the passes of a multi-pass chain: 0 automatic,
+                         1 register-resident two-stage passes where compiled,
+                         2 LDS Stockham super-stages only */
 } flagfftxTuning;
 
 flagfftResult flagfftxSetTuning(const flagfftxTuning *tuning); /* NULL restores defaults */

@@ -249,7 +249,8 @@ def test_planner_choices(untuned):
     assert "RealEvenFused(n=6000" in _describe(6000, T.R2C)
     assert "RealEvenSplitPass(n=32768" in _describe(32768, T.C2R)
     assert "RealPromoted(n=243" in _describe(243, T.D2Z)
-    assert "super_radices=[4096,4096]" in _describe(1 << 24)
+    assert "super_radices=[256,256,256]" in _describe(1 << 24)   # register-resident passes beat [4096,4096]
+    assert "super_radices=[1024,1024]" in _describe(1 << 20)
     assert "TwoDim(n0=4096, n1=4096" in _describe([4096, 4096], T.R2C, rank=2)
 
 
