@@ -676,7 +676,7 @@ def run_workload(spec, args, torch, ff, dev, stream, seed):
     for i, c in enumerate(cases):
         c.allocate(nsets, seed + i)
     torch.cuda.synchronize()
-    nq = min(hw_queues(), len(cases))
+    nq = min(args.streams if args.streams > 0 else hw_queues(), len(cases))
     slot, alone_ms = assign_streams(cases, max(args.warmup, 3), torch, stream, nq)
     if nq > 1:
         case_streams, slot, stream_table = pick_streams(cases, alone_ms, torch, dev, nq)
@@ -717,6 +717,9 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="measure the headline workload only")
     ap.add_argument("--suite", action="store_true", help="also measure further configurations (stderr)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams for the families (0: calibrate over 1 to the hardware-queue count; 1: one stream, "
+                         "every kernel alone on the GPU, as in the per-launch profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes 20x "
                     "(counter calibration run under rocprofv3)")
     ap.add_argument("--write-traffic", help="write the per-kernel HBM traffic summary to this file and exit")

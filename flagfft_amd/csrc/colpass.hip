@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 
 #include "codelets.hpp"
 #include "dev_common.hpp"
@@ -39,6 +40,14 @@ template <int C, int RB>
 __device__ __forceinline__ int xslot(int kk, int t, int sl) {
   const int idx = (kk * RB + t) * C + sl;
   return idx + (idx >> 5);
+}
+
+// Store to element `idx` of an array with a uniform base and a 32-bit lane
+// offset (global_store with saddr), as ldt loads.
+template <typename T>
+__device__ __forceinline__ void stt(GlbC<T> *base, uint32_t idx, V2<T> v) {
+  FLAGFFT_GLB char *b = reinterpret_cast<FLAGFFT_GLB char *>(base);
+  stg(reinterpret_cast<GlbC<T> *>(b + idx * static_cast<uint32_t>(sizeof(V2<T>))), v);
 }
 
 // p[j] = w^j for j < 8 (p[0] = 1), each at most three products deep.
@@ -73,17 +82,18 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
   const int sl = tid % C;
   const int t = tid / C;
   const int blk = remap_block(blockIdx.x, a.nblocks, a.xcd_remap);
+  // The host launches this kernel only when C divides tcnt (every tile lies
+  // in one sequence) and a sequence spans under 4 GiB: the sequence base is
+  // uniform and every element offset fits 32 bits.
   const int64_t u0 = static_cast<int64_t>(blk) * C;
-  const int64_t u = u0 + sl;
-  const bool active = u < a.units;
-  const uint32_t uu = active ? static_cast<uint32_t>(u) : 0u;
-  const uint32_t bq = fdiv(uu, a.fd_tcnt);
-  const uint32_t i = uu - bq * static_cast<uint32_t>(a.tcnt);
+  const bool active = u0 + sl < a.units;
+  const uint32_t bq = fdiv(static_cast<uint32_t>(u0), a.fd_tcnt);
+  const uint32_t i = static_cast<uint32_t>(u0) - bq * static_cast<uint32_t>(a.tcnt) + sl;
   const bool first = a.p == a.ncol;
   const uint32_t k = i - fdiv(i, a.fd_p) * static_cast<uint32_t>(a.p);
   const uint32_t kt = first ? 0u : fdiv(k, a.fd_ncol);
-  const int64_t tcnt = a.tcnt;
-  const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in + i;
+  const uint32_t tcnt = static_cast<uint32_t>(a.tcnt);
+  const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in;
   const GlbC<T> *tws = gc<T>(a.tw_small);
   const GlbC<T> *tw_lo = gc<T>(a.tw_lo);
   const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
@@ -95,8 +105,8 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
   V2<T> x[RA];
 #pragma unroll
   for (int j = 0; j < RA; ++j) {
-    const int64_t e = t + static_cast<int64_t>(RB) * j;
-    x[j] = active ? ldg(src + e * tcnt) : V2<T>{0, 0};
+    const uint32_t e = static_cast<uint32_t>(t + RB * j);
+    x[j] = active ? ldt(src, i + e * tcnt) : V2<T>{0, 0};
   }
   constexpr int L8 = RA < 8 ? RA : 8;
   constexpr int H8 = (RA + 7) / 8;
@@ -209,9 +219,9 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
     });
     if (bs) {
       // unit u0 + rsl owns outputs (u0 + rsl) R + e', e' = rk1 + RA k2
-      const int64_t uo = u0 + rsl;
-      if (uo < a.units) {
-        GlbC<T> *dst = gc<T>(a.out) + uo * R + rk1;
+      if (u0 + rsl < a.units) {
+        GlbC<T> *dst = gc<T>(a.out) + u0 * R;
+        const uint32_t o0 = static_cast<uint32_t>(rsl * R + rk1);
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
           V2<T> v = y[0][q];
@@ -220,7 +230,7 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
             v.x *= scale;
             v.y *= scale;
           }
-          stg(dst + RA * q, v);
+          stt(dst, o0 + static_cast<uint32_t>(RA * q), v);
         }
       }
       return;
@@ -289,14 +299,16 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
       }
     }
   } else if (active) {
-    GlbC<T> *dst = gc<T>(a.out) + static_cast<int64_t>(bq) * a.bs_out + (static_cast<int64_t>(i - k) * R + k);
+    GlbC<T> *dst = gc<T>(a.out) + static_cast<int64_t>(bq) * a.bs_out;
+    const uint32_t o0 = (i - k) * static_cast<uint32_t>(R) + k;
+    const uint32_t pp = static_cast<uint32_t>(a.p);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       const int k1 = t + RB * m;
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
-        const int64_t e2 = k1 + static_cast<int64_t>(RA) * q;
-        stg(dst + e2 * a.p, finish(y[m][q]));
+        const uint32_t e2 = static_cast<uint32_t>(k1 + RA * q);
+        stt(dst, o0 + e2 * pp, finish(y[m][q]));
       }
     }
   }
@@ -307,47 +319,56 @@ __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) v
 namespace {
 
 struct ColEntry {
-  int r, ra, rb, c, nch;
+  int r, ra, rb, c, nch, wpe;
   bool dbl;
   const void *f;
+  int pref;  // default instance for first passes (1), later passes (2), both (3), neither (0)
 };
 
-#define FLAGFFT_COL(T, C, RA, RB, NCH, WPE) \
-  {RA * RB, RA, RB, C, NCH, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1>}
+#define FLAGFFT_COL(T, C, RA, RB, NCH, WPE, PREF) \
+  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1>, PREF}
 
-// Earlier entries of one (R, precision) are the defaults.
+// Measured on MI355X (2^20 x 16 and 2^24 C2C FP32): for R = 1024 the first
+// pass (contiguous stores) runs best with 32 units per workgroup, later
+// passes (C-element stores) with 16; for R = 256, 16 units throughout.
 const ColEntry kColTable[] = {
-    FLAGFFT_COL(float, 16, 32, 32, 2, 4),    // 1024: 128-byte runs, 66 KB LDS, two workgroups per CU
-    FLAGFFT_COL(float, 16, 32, 32, 1, 2),    // 1024: 132 KB LDS, one workgroup per CU
-    FLAGFFT_COL(float, 8, 32, 32, 1, 2),     // 1024: 64-byte runs
-    FLAGFFT_COL(float, 32, 16, 16, 1, 4),    // 256: 256-byte runs, 66 KB LDS
-    FLAGFFT_COL(float, 64, 16, 16, 2, 4),    // 256: 512-byte runs, one workgroup per CU
-    FLAGFFT_COL(float, 16, 16, 16, 1, 2),    // 256: 128-byte runs, 33 KB LDS
-    FLAGFFT_COL(float, 16, 32, 16, 1, 2),    // 512: 128-byte runs
-    FLAGFFT_COL(float, 32, 16, 8, 1, 2),     // 128
-    FLAGFFT_COL(float, 64, 8, 8, 1, 4),      // 64: 512-byte runs
-    FLAGFFT_COL(float, 64, 8, 4, 1, 2),      // 32: 512-byte runs
-    FLAGFFT_COL(float, 64, 4, 4, 1, 2),      // 16: 512-byte runs
-    FLAGFFT_COL(float, 32, 32, 32, 2, 4),    // 1024: 256-byte runs, 132 KB LDS, 1024 threads
-    FLAGFFT_COL(double, 16, 16, 16, 1, 2),   // 256: 256-byte runs, 66 KB LDS
-    FLAGFFT_COL(double, 8, 16, 16, 1, 2),    // 256: 128-byte runs
-    FLAGFFT_COL(double, 16, 16, 8, 1, 2),    // 128
-    FLAGFFT_COL(double, 32, 8, 8, 1, 2),     // 64
+    FLAGFFT_COL(float, 16, 32, 32, 2, 4, 2),    // 1024: 128-byte runs, 68 KB LDS, two workgroups per CU
+    FLAGFFT_COL(float, 32, 32, 32, 2, 4, 1),    // 1024: 256-byte runs, 136 KB LDS, 1024 threads
+    FLAGFFT_COL(float, 16, 32, 32, 1, 2, 0),    // 1024: 132 KB LDS, one workgroup per CU
+    FLAGFFT_COL(float, 8, 32, 32, 2, 4, 0),     // 1024: 64-byte runs, 35 KB LDS, four workgroups per CU
+    FLAGFFT_COL(float, 8, 32, 32, 1, 2, 0),     // 1024: 64-byte runs
+    FLAGFFT_COL(float, 16, 16, 16, 1, 2, 3),    // 256: 128-byte runs, 33 KB LDS
+    FLAGFFT_COL(float, 32, 16, 16, 1, 4, 0),    // 256: 256-byte runs, 66 KB LDS
+    FLAGFFT_COL(float, 64, 16, 16, 2, 4, 0),    // 256: 512-byte runs, one workgroup per CU
+    FLAGFFT_COL(float, 16, 32, 16, 1, 2, 3),    // 512: 128-byte runs
+    FLAGFFT_COL(float, 32, 16, 8, 1, 2, 3),     // 128
+    FLAGFFT_COL(float, 64, 8, 8, 1, 4, 3),      // 64: 512-byte runs
+    FLAGFFT_COL(float, 64, 8, 4, 1, 2, 3),      // 32: 512-byte runs
+    FLAGFFT_COL(float, 64, 4, 4, 1, 2, 3),      // 16: 512-byte runs
+    FLAGFFT_COL(double, 16, 16, 16, 1, 2, 3),   // 256: 256-byte runs, 66 KB LDS
+    FLAGFFT_COL(double, 8, 16, 16, 1, 2, 0),    // 256: 128-byte runs
+    FLAGFFT_COL(double, 16, 16, 8, 1, 2, 3),    // 128
+    FLAGFFT_COL(double, 32, 8, 8, 1, 2, 3),     // 64
 };
 
 }  // namespace
 
-// units: 0 for the default instance, C for the first instance with C units
-// per workgroup, 1000 NCH + C for the one with NCH exchange chunks as well.
-int col_kernel_id(int64_t r, bool dbl, int units) {
+// units: 0 for the default instance of a first (first = true) or later
+// pass, C for the first instance with C units per workgroup, 1000 NCH + C
+// for the one with NCH exchange chunks as well.
+int col_kernel_id(int64_t r, bool dbl, int units, bool first) {
   const int n = static_cast<int>(sizeof(kColTable) / sizeof(kColTable[0]));
   const int c = units % 1000, nch = units / 1000;
+  const int want = first ? 1 : 2;
   for (int id = 0; id < n; ++id) {
     const ColEntry &e = kColTable[id];
-    if (e.r == r && e.dbl == dbl && (units <= 0 || (e.c == c && (nch == 0 || e.nch == nch)))) return id;
+    if (e.r != r || e.dbl != dbl) continue;
+    if (units <= 0 ? (e.pref & want) != 0 : (e.c == c && (nch == 0 || e.nch == nch))) return id;
   }
   return -1;
 }
+
+int col_kernel_id(int64_t r, bool dbl, int units) { return col_kernel_id(r, dbl, units, false); }
 
 void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes) {
   const ColEntry &e = kColTable[id];
@@ -367,5 +388,11 @@ void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &
 }
 
 const void *col_kernel(int id) { return kColTable[id].f; }
+
+std::string col_kernel_targs(int id) {
+  const ColEntry &e = kColTable[id];
+  return std::to_string(e.c) + "," + std::to_string(e.ra) + "," + std::to_string(e.rb) + "," + std::to_string(e.nch) +
+         "," + std::to_string(e.wpe) + ",1";
+}
 
 }  // namespace flagfft

@@ -746,7 +746,7 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   // register-resident two-stage pass (plain complex chain passes)
   const int cid = (mode == 0 && opt_.stage_form != 2 && !opt_.generic_kernels && a.src_mode == kSrcComplex &&
                    a.dst_mode == kDstComplex)
-                      ? col_kernel_id(a.R, dbl(), opt_.forced_units)
+                      ? col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1)
                       : -1;
   if ((sid < 0 && cid < 0) || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
   // contiguous sequences, or the columns of contiguous rows of ncol elements
@@ -758,10 +758,14 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   const int R = a.R;
   const int64_t tcnt = a.tcnt * ncol, p = a.p * ncol;
   if (tcnt >= (int64_t{1} << 31)) return false;
-  if (cid >= 0) {
-    int cunits = 0, cthreads = 0, cebytes = 0;
-    size_t clds = 0;
-    col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
+  int cunits = 0, cthreads = 0, cebytes = 0;
+  size_t clds = 0;
+  if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
+  // the register-resident pass addresses a sequence from a uniform base with
+  // 32-bit offsets: its tiles may not straddle sequences, nor a sequence 4 GiB
+  const bool col_ok = cid >= 0 && tcnt % cunits == 0 && tcnt * R * eb() < (int64_t{1} << 32);
+  if (!col_ok && sid < 0) return false;
+  if (col_ok) {
     Step cst;
     cst.kind = Step::Col;
     cst.dbl = dbl();
@@ -798,7 +802,7 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
     cst.bytes_moved =
         s.nbatch * ncol * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
     std::ostringstream lab;
-    lab << "k_col<" << (dbl() ? "fp64" : "fp32") << "," << cunits << "> pass " << pass << "/" << npasses
+    lab << "k_col<" << (dbl() ? "fp64" : "fp32") << "," << col_kernel_targs(cid) << "> pass " << pass << "/" << npasses
         << " R=" << R << " p=" << p << " tcnt=" << tcnt << (g.block_store ? " block-store" : "")
         << (cols ? " columns=" : "");
     if (cols) lab << ncol;
@@ -806,7 +810,6 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
     ex_.steps.push_back(cst);
     return true;
   }
-  if (sid < 0) return false;
   int cu = 0, ept = 0;
   stage_kernel_shape(sid, dbl(), cu, ept);
   const int tpu = (R + ept - 1) / ept;

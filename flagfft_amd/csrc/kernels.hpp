@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "fft_args.hpp"
@@ -49,8 +50,11 @@ const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 
 // super-radix r with `units` per workgroup (0: the default one), -1 if none;
 // its units, threads, dynamic LDS bytes and element bytes; the kernel.
 int col_kernel_id(int64_t r, bool dbl, int units);
+int col_kernel_id(int64_t r, bool dbl, int units, bool first);
 void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes);
 const void *col_kernel(int id);
+// Template arguments after the element type, as the kernel symbol lists them.
+std::string col_kernel_targs(int id);
 
 // Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
 // QueueArgs argument).
