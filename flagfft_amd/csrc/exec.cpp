@@ -744,10 +744,11 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
                         const SeqSet &s, int64_t n, int pass, int npasses) {
   const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
   // register-resident two-stage pass (plain complex chain passes)
-  const int cid = (mode == 0 && opt_.stage_form != 2 && !opt_.generic_kernels && a.src_mode == kSrcComplex &&
-                   a.dst_mode == kDstComplex)
-                      ? col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1)
-                      : -1;
+  const bool col_allowed = mode == 0 && opt_.stage_form != 2 && (!opt_.generic_kernels || opt_.queue_form) &&
+                           a.src_mode == kSrcComplex && a.dst_mode == kDstComplex;
+  const int cid = !col_allowed ? -1
+                  : opt_.queue_form ? col_queue_kernel_id(a.R, dbl())
+                                    : col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1);
   if ((sid < 0 && cid < 0) || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
   // contiguous sequences, or the columns of contiguous rows of ncol elements
   const bool rows = !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1;

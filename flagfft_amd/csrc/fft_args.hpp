@@ -219,25 +219,6 @@ struct MfmaArgs {
   double scale;
 };
 
-// One job of the persistent mixed-length kernel: a single-pass transform of
-// `nblocks` workgroup tiles whose arguments are those of the equivalent
-// standalone launch (FP32 kernels with 16 elements per thread, FP64 with 8).
-enum QueueKind : int32_t { kQueuePass = 0, kQueueRader = 1, kQueueBluestein = 2, kQueueR2CEven = 3, kQueueC2REven = 4 };
-struct QueueJob {
-  int32_t kind;     // QueueKind
-  int32_t dbl;
-  int32_t first;    // index of the job's first tile in the launch
-  int32_t nblocks;  // tiles
-  PassArgs pa;      // kQueuePass
-  FusedArgs fa;     // the other kinds
-};
-struct QueueArgs {
-  const QueueJob *jobs;  // sorted by `first`
-  int32_t njobs;
-  int32_t total;         // tiles over all jobs
-  uint32_t *counter;     // next tile to hand out; 0 at launch
-};
-constexpr int kQueueThreads = 256;
 
 // Split-spectrum pre/post pass for even-length real transforms that do not fit one LDS transform.
 struct RealPassArgs {
@@ -250,5 +231,43 @@ struct RealPassArgs {
   const void *tw_half;  // w_n^k, k <= n/2
   int32_t conj_flag;
 };
+
+// One job of the persistent mixed-length kernel: `nblocks` workgroup tiles of
+// one launch of a plan (a single-pass transform, or one pass of a multi-pass
+// chain whose earlier passes ran in an earlier queue launch), with the
+// arguments of the equivalent standalone launch (run-time-radix kinds: FP32
+// with 16 elements per thread, FP64 with 8; register-resident passes: the
+// queue instance `variant` of 256 threads).
+enum QueueKind : int32_t {
+  kQueuePass = 0,
+  kQueueRader = 1,
+  kQueueBluestein = 2,
+  kQueueR2CEven = 3,
+  kQueueC2REven = 4,
+  kQueueCol = 5,
+  kQueueRealPost = 6,
+  kQueueRealPre = 7
+};
+struct QueueJob {
+  int32_t kind;     // QueueKind
+  int32_t dbl;
+  int32_t first;    // index of the job's first tile in the launch
+  int32_t nblocks;  // tiles
+  int32_t variant;  // kQueueCol: queue instance of the register-resident pass
+  int32_t pad_;
+  union {
+    PassArgs pa;      // kQueuePass
+    FusedArgs fa;     // kQueueRader, kQueueBluestein, kQueueR2CEven, kQueueC2REven
+    StageArgs sa;     // kQueueCol
+    RealPassArgs ra;  // kQueueRealPost, kQueueRealPre
+  };
+};
+struct QueueArgs {
+  const QueueJob *jobs;  // sorted by `first`
+  int32_t njobs;
+  int32_t total;         // tiles over all jobs
+  uint32_t *counter;     // next tile to hand out; 0 at launch
+};
+constexpr int kQueueThreads = 256;
 
 }  // namespace flagfft

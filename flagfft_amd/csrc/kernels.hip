@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "codelets.hpp"
+#include "colpass_body.hpp"
 #include "dev_common.hpp"
 #include "fft_args.hpp"
 #include "kernels.hpp"
@@ -650,13 +651,19 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_c2r_even(const FusedArgs a
   c2r_even_body<T, E>(a, blockIdx.x);
 }
 
+template <typename T>
+__device__ __forceinline__ void real_post_body(const RealPassArgs &a, int blk, int nblk);
+template <typename T>
+__device__ __forceinline__ void real_pre_body(const RealPassArgs &a, int blk, int nblk);
+
 // ---------------------------------------------------------------------------
 // Persistent mixed-length kernel.  A fixed grid of workgroups drains a queue of
-// tiles from single-pass jobs of any length, type and precision: each
-// workgroup takes the next tile index from a global counter, finds its job by
-// binary search over the jobs' first tiles and runs the same device body as
-// the job's standalone kernel (so results are bit-identical to it).  Every
-// workgroup exits once the counter passes the last tile.
+// tiles from jobs of any length, type and precision (single-pass transforms,
+// or one pass of multi-pass chains): each workgroup takes the next tile index
+// from a global counter, finds its job by binary search over the jobs' first
+// tiles and runs the same device body as the job's standalone kernel (so
+// results are bit-identical to it).  Every workgroup exits once the counter
+// passes the last tile.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kQueueThreads) void k_queue(const QueueArgs q) {
   __shared__ int item_s;
@@ -664,7 +671,7 @@ __global__ __launch_bounds__(kQueueThreads) void k_queue(const QueueArgs q) {
     __syncthreads();  // the previous tile is done with LDS and item_s
     if (threadIdx.x == 0) item_s = static_cast<int>(atomicAdd(q.counter, 1u));
     __syncthreads();
-    const int item = item_s;
+    const int item = __builtin_amdgcn_readfirstlane(item_s);  // uniform: the job's fields load as scalars
     if (item >= q.total) break;
     int lo = 0, hi = q.njobs - 1;
     while (lo < hi) {
@@ -685,8 +692,52 @@ __global__ __launch_bounds__(kQueueThreads) void k_queue(const QueueArgs q) {
       case kQueueR2CEven * 2 + 1: r2c_even_body<double, 8>(j.fa, blk); break;
       case kQueueC2REven * 2: c2r_even_body<float, 16>(j.fa, blk); break;
       case kQueueC2REven * 2 + 1: c2r_even_body<double, 8>(j.fa, blk); break;
+      case kQueueRealPost * 2: real_post_body<float>(j.ra, blk, j.nblocks); break;
+      case kQueueRealPost * 2 + 1: real_post_body<double>(j.ra, blk, j.nblocks); break;
+      case kQueueRealPre * 2: real_pre_body<float>(j.ra, blk, j.nblocks); break;
+      case kQueueRealPre * 2 + 1: real_pre_body<double>(j.ra, blk, j.nblocks); break;
       default: break;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent kernels of the register-resident passes: one per instance (the
+// queue variants of colpass.hip's catalogue), each draining the jobs of that
+// instance with k_queue's tile loop, so that every instance keeps its own
+// register count (a kernel holding several would need the registers of all).
+// ---------------------------------------------------------------------------
+template <typename T, int C, int RA, int RB, int NCH, int WPE>
+__global__ __launch_bounds__(kQueueThreads) __attribute__((amdgpu_waves_per_eu(WPE))) void k_queue_col(const QueueArgs q) {
+  __shared__ int item_s;
+  for (;;) {
+    __syncthreads();  // the previous tile is done with LDS and item_s
+    if (threadIdx.x == 0) item_s = static_cast<int>(atomicAdd(q.counter, 1u));
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(item_s);  // uniform: the job's fields load as scalars
+    if (item >= q.total) break;
+    int lo = 0, hi = q.njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (q.jobs[mid].first <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    // The job's arguments in scalar registers, as a standalone launch has its
+    // kernel arguments: lane i loads dword i, readlane broadcasts each one
+    // (a plain copy from the job array would sit in vector registers).
+    constexpr int NW = static_cast<int>(sizeof(StageArgs) / 4);
+    static_assert(NW <= 64 && sizeof(StageArgs) % 4 == 0, "StageArgs must fit one wavefront's lanes");
+    union {
+      StageArgs a;
+      uint32_t w[NW];
+    } u;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&q.jobs[lo].sa);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const uint32_t mine = lane < NW ? src[lane] : 0u;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) u.w[w] = __builtin_amdgcn_readlane(mine, w);
+    const int first = __builtin_amdgcn_readfirstlane(q.jobs[lo].first);
+    col_body<T, C, RA, RB, NCH, 1>(u.a, item - first);
   }
 }
 
@@ -694,14 +745,15 @@ __global__ __launch_bounds__(kQueueThreads) void k_queue(const QueueArgs q) {
 // Split-spectrum passes for even-length real transforms that run multi-pass:
 //   post: Z (h complex, contiguous) -> X (h+1 bins);  pre: X -> conj(Z').
 // ---------------------------------------------------------------------------
+// Tile blk of nblk (256 threads each) strides over the elements.
 template <typename T>
-__global__ void k_real_post(const RealPassArgs a) {
+__device__ __forceinline__ void real_post_body(const RealPassArgs &a, int blk, int nblk) {
   const int h = a.n / 2;
   const int64_t total = a.nseqs * (h + 1);
   const GlbC<T> *wn = gc<T>(a.tw_half);
   const T half = T(0.5);
-  for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
+       g += static_cast<int64_t>(nblk) * kQueueThreads) {
     const int64_t q = g / (h + 1);
     const int k = static_cast<int>(g - q * (h + 1));
     const int64_t b = q / a.ncol, c = q - b * a.ncol;
@@ -715,12 +767,17 @@ __global__ void k_real_post(const RealPassArgs a) {
 }
 
 template <typename T>
-__global__ void k_real_pre(const RealPassArgs a) {
+__global__ __launch_bounds__(kQueueThreads) void k_real_post(const RealPassArgs a) {
+  real_post_body<T>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__device__ __forceinline__ void real_pre_body(const RealPassArgs &a, int blk, int nblk) {
   const int h = a.n / 2;
   const int64_t total = a.nseqs * h;
   const GlbC<T> *wn = gc<T>(a.tw_half);
-  for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
+       g += static_cast<int64_t>(nblk) * kQueueThreads) {
     const int64_t q = g / h;
     const int k = static_cast<int>(g - q * h);
     const int64_t b = q / a.ncol, c = q - b * a.ncol;
@@ -735,6 +792,11 @@ __global__ void k_real_pre(const RealPassArgs a) {
     const V2<T> t = cmul_pi(cmul(csub(xk, xh), cconj(ldg(wn + k))));
     stg(gc<T>(a.out) + b * a.go.bs + c * a.go.cs + k * a.go.es, cconj(cadd(cadd(xk, xh), t)));
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kQueueThreads) void k_real_pre(const RealPassArgs a) {
+  real_pre_body<T>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -860,6 +922,22 @@ const void *r2c_even_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_r2c_even);
 const void *c2r_even_kernel(bool dbl, int ept) { FLAGFFT_KERNEL_EPT(k_c2r_even); }
 
 const void *queue_kernel() { return reinterpret_cast<const void *>(dev::k_queue); }
+// Indexed by QueueJob::variant (colpass.hip's queue variants); null where unused.
+const void *queue_col_kernel(int variant) {
+  static const void *const k[] = {
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 8, 32, 32, 2, 4>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 16, 32, 16, 1, 2>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 16, 16, 16, 1, 4>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 32, 16, 8, 1, 3>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 32, 8, 8, 1, 4>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 64, 8, 4, 1, 4>),
+      reinterpret_cast<const void *>(dev::k_queue_col<float, 64, 4, 4, 1, 4>),
+      reinterpret_cast<const void *>(dev::k_queue_col<double, 16, 16, 16, 1, 2>),
+      nullptr,
+      reinterpret_cast<const void *>(dev::k_queue_col<double, 32, 8, 8, 1, 4>),
+  };
+  return variant >= 0 && variant < static_cast<int>(sizeof(k) / sizeof(k[0])) ? k[variant] : nullptr;
+}
 
 const void *real_post_kernel(bool dbl) {
   return dbl ? reinterpret_cast<const void *>(dev::k_real_post<double>)

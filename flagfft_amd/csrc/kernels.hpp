@@ -53,12 +53,19 @@ int col_kernel_id(int64_t r, bool dbl, int units);
 int col_kernel_id(int64_t r, bool dbl, int units, bool first);
 void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes);
 const void *col_kernel(int id);
+// The instance of 256 threads the persistent queue kernel also runs (-1 if
+// none for r), and its queue variant (QueueJob::variant; -1 for others).
+int col_queue_kernel_id(int64_t r, bool dbl);
+int col_queue_variant(int id);
 // Template arguments after the element type, as the kernel symbol lists them.
 std::string col_kernel_targs(int id);
 
 // Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
 // QueueArgs argument).
 const void *queue_kernel();
+// Persistent kernel of the register-resident pass jobs (kQueueCol) of one
+// queue variant (QueueJob::variant), null if none.
+const void *queue_col_kernel(int variant);
 
 // Bluestein kernels with the convolution in LDS (FP32): id for the
 // convolution's radix sequence (-1 if none), elements per thread, handle.

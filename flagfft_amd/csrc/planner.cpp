@@ -266,7 +266,8 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   int64_t forced_prod = o.forced_passes.size() >= 2 ? 1 : 0;
   for (int64_t r : o.forced_passes) forced_prod *= r;
   bool leaf = n <= lmax && radix_factor(n, rad) && forced_prod != n;
-  if (!leaf && o.forced_passes.empty() && n <= 32768 && radix_factor(n, rad) && leaf_kernel_id(rad) >= 0 &&
+  if (!leaf && !o.generic_kernels && o.forced_passes.empty() && n <= 32768 && radix_factor(n, rad) &&
+      leaf_kernel_id(rad) >= 0 &&
       static_cast<int64_t>(lds_pad(static_cast<int>(n) - 1) + 2) * elem_bytes(p) <= o.lds_cap_bytes) {
     // a specialised instance holding the whole sequence in LDS: one HBM round
     // trip instead of two (one workgroup per sequence)

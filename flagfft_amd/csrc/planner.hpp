@@ -37,6 +37,7 @@ struct PlanOptions {
   int stage_form = 0;                  // multi-pass chains: 0 automatic, 1 register-resident passes, 2 LDS super-stages
   int max_threads = 512;               // workgroup size limit of the launches
   bool generic_kernels = false;        // only the run-time-radix kernels (persistent-queue jobs)
+  bool queue_form = false;             // with generic_kernels: register-resident passes the queue kernel runs
   bool in_place = false;               // input and output are one buffer: no sequence split over workgroups
 };
 

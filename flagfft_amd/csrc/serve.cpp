@@ -14,6 +14,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -37,7 +38,13 @@ using Clock = std::chrono::steady_clock;
 
 constexpr size_t kAlign = 256;
 constexpr int kMaxJobs = 2048;                     // queue jobs per persistent launch
-constexpr int64_t kQueueMaxBytes = int64_t{32} << 20;  // larger single-pass requests launch on their own
+constexpr int64_t kQueueMaxBytes = int64_t{32} << 20;   // larger requests with run-time-radix passes launch on their own
+constexpr int64_t kQueueMaxBytesFast = int64_t{1} << 30;  // the same for plans of register-resident passes
+constexpr int kMaxQueueSteps = 6;                       // queue launches per batch (passes of the longest plan)
+// queue launches per batch: per step, the run-time-radix kernel and one per
+// register-resident variant present (at most 10)
+constexpr int kMaxQueueLaunches = kMaxQueueSteps * 11;
+constexpr size_t kStagingBytes = 256 * kMaxQueueLaunches + sizeof(QueueJob) * kMaxJobs * kMaxQueueSteps;
 constexpr int64_t kNominalBatch = 512;             // batch the cached plans are shaped for
 constexpr double kLargeRequestBytes = 64.0 * (1 << 20);  // direct requests above this stay in arrival order
 constexpr int64_t kQueueNominal = int64_t{1} << 20;  // queue templates: full tiles (C * tpu ~ 256 threads)
@@ -104,8 +111,14 @@ struct ServePlan {
   int64_t in_bytes = 0, out_bytes = 0;      // per transform
   std::unique_ptr<Executable> plain;        // specialised kernels, caller workspaces
   std::vector<size_t> ws_per_transform;     // plain: workspace bytes per transform
-  std::unique_ptr<Executable> generic;      // one run-time-radix step usable as a queue job, or null
-  int queue_kind = -1;
+  // The plan as persistent-queue jobs: every launch of it has a queue kind
+  // (run-time-radix bodies, register-resident passes of 256 threads, real
+  // split passes); step s of a batch's requests runs in the batch's s-th
+  // queue launch.  Null when some launch has no queue form.
+  std::unique_ptr<Executable> queue;
+  std::vector<int> queue_kinds, queue_variants;  // per step
+  std::vector<size_t> queue_ws_per_transform;
+  bool queue_fast = false;  // no run-time-radix pass over more than one tile column (see kQueueMaxBytesFast)
 };
 
 struct Request {
@@ -132,12 +145,24 @@ struct Slot {
   bool busy = false;  // job staging in use by a queue launch
   double load = 0.0;  // algorithmic bytes launched and not yet completed
   std::vector<Group> groups;
-  QueueJob *host_jobs = nullptr;  // pinned staging: counter block then jobs
+  // pinned staging of the batch's queue launches: each launch's counter block
+  // (256 bytes), then its jobs, one launch after the other
+  QueueJob *host_jobs = nullptr;
   void *dev_jobs = nullptr;
 };
 
-bool queue_kind_of(const Step &s, int &kind) {
+bool queue_kind_of(const Step &s, int &kind, int &variant) {
+  variant = 0;
   switch (s.kind) {
+    case Step::Col:
+      kind = kQueueCol;
+      variant = col_queue_variant(s.leaf_id);
+      return variant >= 0 && s.shape.block.x == static_cast<unsigned>(kQueueThreads) &&
+             s.shape.lds_bytes <= 80 * 1024;
+    case Step::RealPost:
+    case Step::RealPre:
+      kind = s.kind == Step::RealPost ? kQueueRealPost : kQueueRealPre;
+      return s.shape.block.x == static_cast<unsigned>(kQueueThreads) && s.shape.lds_bytes == 0;
     case Step::Pass: kind = kQueuePass; break;
     case Step::Rader: kind = kQueueRader; break;
     case Step::Bluestein: kind = kQueueBluestein; break;
@@ -163,7 +188,7 @@ class Server {
     hip_check(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
     ncu_ = prop.multiProcessorCount;
     slots_.resize(static_cast<size_t>(nslots_));
-    const size_t job_bytes = kAlign + sizeof(QueueJob) * kMaxJobs;
+    const size_t job_bytes = kStagingBytes;
     for (Slot &s : slots_) {
       hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "server stream");
       if (mode_ == FLAGFFTX_SERVE_BATCHED) {
@@ -231,21 +256,48 @@ class Server {
       if (mode_ == FLAGFFTX_SERVE_BATCHED) {
         PlanOptions g;
         g.generic_kernels = true;
+        g.queue_form = true;
         g.max_threads = kQueueThreads;
         g.ept = dbl ? 8 : 16;
+        // single-pass queue jobs hold a whole sequence in one 256-thread tile;
+        // longer lengths run as chains of queue passes
+        g.leaf_max_f32 = int64_t{kQueueThreads} * 16;
+        g.leaf_max_f64 = int64_t{kQueueThreads} * 8;
         try {
           PlanDesc dg = d;
           dg.batch = kQueueNominal;
           PlanTree tg = plan_tree(dg, g);
           auto ex = build_executable(tg, inverse, g, in_len, out_len, false);
-          int kind = -1;
-          if (ex->steps.size() == 1 && ex->ws_bytes.empty() && queue_kind_of(ex->steps[0], kind)) {
-            sp->queue_kind = kind;
-            table_bytes_ += tables_of(*ex);
-            sp->generic = std::move(ex);
+          bool ok = !ex->steps.empty() && ex->steps.size() <= static_cast<size_t>(kMaxQueueSteps);
+          bool fast = true;
+          std::vector<int> kinds, variants;
+          const bool explain = std::getenv("FLAGFFT_SERVE_EXPLAIN") != nullptr;
+          for (const Step &s : ex->steps) {
+            int kind = -1, variant = 0;
+            const bool q = queue_kind_of(s, kind, variant);
+            if (!q && explain)
+              std::fprintf(stderr, "[flagfft] type %d n=%lld: no queue form for launch '%s' (%u threads, %zu LDS bytes)\n",
+                           type, static_cast<long long>(n), s.label.c_str(), s.shape.block.x, s.shape.lds_bytes);
+            ok = ok && q;
+            kinds.push_back(kind);
+            variants.push_back(variant);
+            // a run-time-radix pass of a multi-pass chain runs at a fraction of the
+            // register-resident rate: such plans stay on the small-request threshold
+            if (s.kind == Step::Pass && ex->steps.size() > 1) fast = false;
           }
-        } catch (const std::exception &) {
+          if (ok) {
+            sp->queue_kinds = kinds;
+            sp->queue_variants = variants;
+            for (size_t b : ex->ws_bytes) sp->queue_ws_per_transform.push_back(b / static_cast<size_t>(kQueueNominal));
+            sp->queue_fast = fast;
+            table_bytes_ += tables_of(*ex);
+            sp->queue = std::move(ex);
+          }
+        } catch (const std::exception &ex) {
           // no queue form (e.g. more than 256 threads per transform): plan launches only
+          if (std::getenv("FLAGFFT_SERVE_EXPLAIN") != nullptr)
+            std::fprintf(stderr, "[flagfft] type %d n=%lld: no queue plan (%s)\n", type, static_cast<long long>(n),
+                         ex.what());
         }
       }
     } catch (const HipError &e) {
@@ -277,12 +329,12 @@ class Server {
     req->in = in;
     req->out_size = static_cast<size_t>(batch * sp->out_bytes);
     size_t need = align_up(req->out_size);
-    req->queued = sp->generic && batch * (sp->in_bytes + sp->out_bytes) <= kQueueMaxBytes;
-    if (!req->queued)
-      for (size_t w : sp->ws_per_transform) {
-        req->ws_size.push_back(w * static_cast<size_t>(batch));
-        need += align_up(req->ws_size.back());
-      }
+    req->queued = sp->queue &&
+                  batch * (sp->in_bytes + sp->out_bytes) <= (sp->queue_fast ? kQueueMaxBytesFast : kQueueMaxBytes);
+    for (size_t w : (req->queued ? sp->queue_ws_per_transform : sp->ws_per_transform)) {
+      req->ws_size.push_back(w * static_cast<size_t>(batch));
+      need += align_up(req->ws_size.back());
+    }
     if (need > arena_.capacity()) return FLAGFFT_ALLOC_FAILED;
     req->arrival = Clock::now();
     {
@@ -538,62 +590,93 @@ class Server {
     for (Request *r : batch) (r->queued && mode_ == FLAGFFTX_SERVE_BATCHED ? queued : direct).push_back(r);
     char *base = arena_.base();
     if (!queued.empty()) {
-      // jobs, largest tiles first, tiles numbered in that order
+      // every request's plan as queue jobs, largest tiles first; launch s runs
+      // step s of every request that has one (stream order keeps a request's
+      // passes in sequence), one event after the last launch
       struct Item {
         Request *r;
-        Step st;
+        std::vector<Step> st;
         double cost;
       };
       std::vector<Item> items;
       items.reserve(queued.size());
+      size_t nsteps = 0;
       for (Request *r : queued) {
-        const Step &t = r->plan->generic->steps[0];
-        Step st = t.rebatched(r->batch, r->plan->generic->built_batch);
-        const double cost = static_cast<double>(r->plan->n) * (r->plan->dbl ? 2.0 : 1.0) * st.shape.block.x;
-        items.push_back({r, st, cost});
+        const Executable &q = *r->plan->queue;
+        Item it{r, {}, 0.0};
+        for (const Step &t : q.steps) it.st.push_back(t.rebatched(r->batch, q.built_batch));
+        it.cost = static_cast<double>(r->plan->n) * (r->plan->dbl ? 2.0 : 1.0) * it.st[0].shape.block.x;
+        nsteps = std::max(nsteps, it.st.size());
+        items.push_back(std::move(it));
       }
       std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
-      uint32_t *counter = reinterpret_cast<uint32_t *>(slot.host_jobs);
-      QueueJob *jobs = reinterpret_cast<QueueJob *>(reinterpret_cast<char *>(slot.host_jobs) + kAlign);
-      int32_t total = 0;
-      size_t lds = 0;
-      for (size_t i = 0; i < items.size(); ++i) {
-        const Item &it = items[i];
-        QueueJob &j = jobs[i];
-        std::memset(&j, 0, sizeof(j));
-        j.kind = it.r->plan->queue_kind;
-        j.dbl = it.r->plan->dbl ? 1 : 0;
-        j.first = total;
-        j.nblocks = static_cast<int32_t>(it.st.shape.grid.x);
-        unsigned char buf[Step::kArgBytes];
-        const size_t nb = it.st.pack_args(const_cast<void *>(it.r->in), base + it.r->out_off, buf);
-        if (j.kind == kQueuePass)
-          std::memcpy(&j.pa, buf, nb);
-        else
-          std::memcpy(&j.fa, buf, nb);
-        total += j.nblocks;
-        lds = std::max(lds, it.st.shape.lds_bytes);
+      size_t cursor = 0;  // staging bytes used by the batch's earlier launches
+      int64_t launches = 0;
+      for (size_t s = 0; s < nsteps; ++s) {
+        // one launch per kernel class present in this step: the run-time-radix
+        // queue kernel (class -1) and each register-resident variant
+        std::vector<int> classes;
+        for (const Item &it : items)
+          if (s < it.st.size()) {
+            const int c = it.r->plan->queue_kinds[s] == kQueueCol ? it.r->plan->queue_variants[s] : -1;
+            if (std::find(classes.begin(), classes.end(), c) == classes.end()) classes.push_back(c);
+          }
+        for (int cls : classes) {
+          char *hreg = reinterpret_cast<char *>(slot.host_jobs) + cursor;
+          char *dreg = static_cast<char *>(slot.dev_jobs) + cursor;
+          uint32_t *counter = reinterpret_cast<uint32_t *>(hreg);
+          QueueJob *jobs = reinterpret_cast<QueueJob *>(hreg + kAlign);
+          int32_t total = 0, njobs = 0;
+          size_t lds = 0;
+          for (const Item &it : items) {
+            if (s >= it.st.size()) continue;
+            Request *r = it.r;
+            const int c = r->plan->queue_kinds[s] == kQueueCol ? r->plan->queue_variants[s] : -1;
+            if (c != cls) continue;
+            const Step &st = it.st[s];
+            auto buf = [&](int id) -> void * {
+              if (id == kBufIn) return const_cast<void *>(r->in);
+              if (id == kBufOut) return base + r->out_off;
+              return base + r->ws_off[static_cast<size_t>(id - kBufWs0)];
+            };
+            QueueJob &j = jobs[njobs++];
+            std::memset(&j, 0, sizeof(j));
+            j.kind = r->plan->queue_kinds[s];
+            j.variant = r->plan->queue_variants[s];
+            j.dbl = r->plan->dbl ? 1 : 0;
+            j.first = total;
+            j.nblocks = static_cast<int32_t>(st.shape.grid.x);
+            unsigned char abuf[Step::kArgBytes];
+            const size_t nb = st.pack_args(buf(st.src), buf(st.dst), abuf, st.aux >= 0 ? buf(st.aux) : nullptr);
+            std::memcpy(&j.pa, abuf, nb);  // the union's members share one address
+            total += j.nblocks;
+            lds = std::max(lds, st.shape.lds_bytes);
+          }
+          *counter = 0;
+          const size_t bytes = kAlign + sizeof(QueueJob) * static_cast<size_t>(njobs);
+          cursor += (bytes + kAlign - 1) / kAlign * kAlign;
+          e = hipMemcpyAsync(dreg, hreg, bytes, hipMemcpyHostToDevice, slot.stream);
+          if (e != hipSuccess) return e;
+          const void *kern = cls < 0 ? queue_kernel() : queue_col_kernel(cls);
+          if (kern == nullptr) return hipErrorInvalidDeviceFunction;
+          QueueArgs qa;
+          qa.jobs = reinterpret_cast<const QueueJob *>(dreg + kAlign);
+          qa.njobs = njobs;
+          qa.total = total;
+          qa.counter = reinterpret_cast<uint32_t *>(dreg);
+          int per_cu = 0;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kQueueThreads, lds) != hipSuccess ||
+              per_cu < 1)
+            per_cu = 1;
+          const int grid = std::max(1, std::min(total, per_cu * ncu_));
+          void *kp[] = {&qa};
+          e = allow_lds(kern, lds);
+          if (e == hipSuccess)
+            e = hipLaunchKernel(kern, dim3(static_cast<unsigned>(grid)), dim3(kQueueThreads), kp, lds, slot.stream);
+          if (e != hipSuccess) return e;
+          ++launches;
+        }
       }
-      *counter = 0;
-      const size_t bytes = kAlign + sizeof(QueueJob) * items.size();
-      e = hipMemcpyAsync(slot.dev_jobs, slot.host_jobs, bytes, hipMemcpyHostToDevice, slot.stream);
-      if (e != hipSuccess) return e;
-      QueueArgs qa;
-      qa.jobs = reinterpret_cast<const QueueJob *>(static_cast<char *>(slot.dev_jobs) + kAlign);
-      qa.njobs = static_cast<int32_t>(items.size());
-      qa.total = total;
-      qa.counter = static_cast<uint32_t *>(slot.dev_jobs);
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, queue_kernel(), kQueueThreads, lds) != hipSuccess ||
-          per_cu < 1)
-        per_cu = 1;
-      const int grid = std::max(1, std::min(total, per_cu * ncu_));
-      void *kp[] = {&qa};
-      e = allow_lds(queue_kernel(), lds);
-      if (e == hipSuccess)
-        e = hipLaunchKernel(queue_kernel(), dim3(static_cast<unsigned>(grid)), dim3(kQueueThreads), kp, lds,
-                            slot.stream);
-      if (e != hipSuccess) return e;
       Group g;
       g.ev = take_event_unlocked();
       g.queue = true;
@@ -605,7 +688,7 @@ class Server {
       groups.push_back(std::move(g));
       if (e != hipSuccess) return e;
       std::lock_guard<std::mutex> lk(mu_);
-      ++st_queue_launches_;
+      st_queue_launches_ += launches;
       st_queued_ += static_cast<int64_t>(items.size());
     }
     for (Request *r : direct) {

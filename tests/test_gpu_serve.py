@@ -247,3 +247,59 @@ def test_serving_trace_poisson_alone_and_in_the_mix(gpu):
             x = flat[: n * b].reshape(b, n)[rows]
         l2, _ = ora.rel_l2(y, _oracle(t, n, x, ff.FORWARD), y.shape[-1])
         assert l2 <= _bound(t, n), f"{t.name} n={n} batch={b} rel_l2={l2:.3e}"
+
+
+# Multi-pass requests whose every pass has a persistent-queue form: register-
+# resident passes of 256 threads (power-of-two super-radices), the real split
+# pass, FP64 passes of 64 and 256 points.  Each request's passes run as
+# queue launches of the batch (step s of every request in the s-th launches).
+MULTI = [
+    (T.C2C, 1 << 16, 3, ff.FORWARD), (T.C2C, 1 << 20, 2, ff.INVERSE), (T.C2C, 1 << 18, 5, ff.FORWARD),
+    (T.C2C, 1 << 14, 9, ff.INVERSE), (T.R2C, 1 << 17, 3, 0), (T.Z2Z, 1 << 16, 2, ff.FORWARD),
+    (T.C2C, 1 << 13, 7, ff.FORWARD),
+]
+
+
+def test_multi_pass_requests_run_in_queue_launches(gpu):
+    """Every MULTI request goes through the persistent queue (one launch per
+    pass and kernel class), gives the same bits alone and in the mix, and
+    matches the FP64 oracle."""
+    import torch
+
+    xs, hosts = [], []
+    for i, (t, n, b, _) in enumerate(MULTI):
+        dbl = t in DBL
+        if t in (T.C2C, T.Z2Z):
+            x = complex_input(900 + i, (b, n), np.complex128 if dbl else np.complex64, "normal")
+        else:
+            x = real_input(900 + i, (b, n), np.float32)
+        hosts.append(x)
+        xs.append(dev_tensor(x, gpu))
+    sync(gpu)
+    with ff.Server(4 << 30, max_inflight=3) as srv:
+        alone = []
+        for x, (t, n, b, d) in zip(xs, MULTI):
+            tk = srv.submit(t, n, b, x.data_ptr(), d)
+            ptr, _ = srv.wait(tk)
+            shape, dt = _out_spec(t, n, b)
+            alone.append(d2h(ptr, shape, dt))
+            srv.release(tk)
+        st_alone = srv.stats()
+        srv.reset_stats()
+        tickets = [srv.submit(t, n, b, x.data_ptr(), d) for x, (t, n, b, d) in zip(xs, MULTI)]
+        mixed = []
+        for tk, (t, n, b, _) in zip(tickets, MULTI):
+            ptr, _ = srv.wait(tk)
+            shape, dt = _out_spec(t, n, b)
+            mixed.append(d2h(ptr, shape, dt))
+            srv.release(tk)
+        st = srv.stats()
+    print(f"multi-pass alone: {st_alone['queued_requests']} queued, {st_alone['queue_launches']} launches; "
+          f"mix: {st['queued_requests']} queued, {st['queue_launches']} launches")
+    assert st_alone["queued_requests"] == len(MULTI) and st["queued_requests"] == len(MULTI), (st_alone, st)
+    # alone, each request takes one launch per pass at least: more launches than requests
+    assert st_alone["queue_launches"] > len(MULTI)
+    for (t, n, b, d), a, m, x in zip(MULTI, alone, mixed, hosts):
+        assert np.array_equal(a, m), f"{t.name} n={n}: result differs between alone and in the mix"
+        l2, _ = ora.rel_l2(m, _oracle(t, n, x, d), m.shape[-1])
+        assert l2 <= _bound(t, n), f"{t.name} n={n} rel_l2={l2:.3e}"
