@@ -48,17 +48,39 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
   const T scale = static_cast<T>(a.scale);
   const GlbC<T> *wn = gc<T>(a.aux);
   const GlbC<T> *tw = gc<T>(a.tw);
+  // split-spectrum passes: thread t handles k = t + i TPU, k <= N/2; every
+  // table (and, for C2R, input) load of them is issued before any is used,
+  // so their latencies overlap instead of one round trip per k
+  constexpr int NK = (N / 2 + TPU) / TPU;
+  V2<T> wk[(Mode == kLeafComplex) ? 1 : NK];
+  if constexpr (Mode != kLeafComplex) {
+#pragma unroll
+    for (int i = 0; i < NK; ++i) {
+      const int k = t + i * TPU;
+      wk[i] = (k <= N / 2) ? ldg(wn + k) : V2<T>{1, 0};
+    }
+  }
   if constexpr (Mode == kLeafC2R) {
     // fused pre-process into LDS (conjugated: the inverse runs as a forward transform)
-    for (int k = t; k <= N / 2; k += TPU) {
-      if (!active) break;
-      V2<T> xk = ldg(src + k);
-      V2<T> xh = ldg(src + (N - k));
+    V2<T> xks[NK], xhs[NK];
+#pragma unroll
+    for (int i = 0; i < NK; ++i) {
+      const int k = t + i * TPU;
+      const bool ok = active && k <= N / 2;
+      xks[i] = ok ? ldg(src + k) : V2<T>{0, 0};
+      xhs[i] = ok ? ldg(src + (N - k)) : V2<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < NK; ++i) {
+      const int k = t + i * TPU;
+      if (!active || k > N / 2) break;
+      V2<T> xk = xks[i];
+      V2<T> xh = xhs[i];
       if (k == 0) {
         xk.y = T(0);
         xh.y = T(0);
       }
-      const V2<T> w = ldg(wn + k);
+      const V2<T> w = wk[i];
       {
         const V2<T> c = cconj(xh);
         const V2<T> z = cadd(cadd(xk, c), cmul_pi(cmul(csub(xk, c), cconj(w))));
@@ -131,13 +153,15 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
       }
       __syncthreads();
       const T half = T(0.5);
-      for (int k = t; k <= N / 2; k += TPU) {
-        if (!active) break;
+#pragma unroll
+      for (int i = 0; i < NK; ++i) {
+        const int k = t + i * TPU;
+        if (!active || k > N / 2) break;
         const V2<T> zk = ldc(buf + lpad(k));
         const V2<T> zc = cconj(ldc(buf + lpad(k == 0 ? 0 : N - k)));
         const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
         const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
-        const V2<T> wo = cmul(ldg(wn + k), od);
+        const V2<T> wo = cmul(wk[i], od);
         V2<T> lo = cadd(ev, wo);
         V2<T> hi = cconj(csub(ev, wo));
         if (a.conj_out) {
