@@ -101,17 +101,30 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
   const T scale = static_cast<T>(a.scale);
 
   // ---- loads: element e = t + RB j of the unit --------------------------------
+  // Bluestein chains: the first pass reads chirp[e] x[e] for e < n (zero past
+  // the user length n = nlog), the last passes write conj(X bhat) (forward
+  // chain) or conj(X) chirp for e < n (inverse chain)
+  const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
+  const uint32_t nlog = static_cast<uint32_t>(a.nlog);
+  const GlbC<T> *chirp = gc<T>(a.chirp);
   V2<T> x[RA];
 #pragma unroll
   for (int j = 0; j < RA; ++j) {
-    const uint32_t e = static_cast<uint32_t>(t + RB * j);
-    x[j] = active ? ldt(src, i + e * tcnt) : V2<T>{0, 0};
+    const uint32_t e = i + static_cast<uint32_t>(t + RB * j) * tcnt;
+    x[j] = (active && (!chirp_in || e < nlog)) ? ldt(src, e) : V2<T>{0, 0};
   }
   constexpr int L8 = RA < 8 ? RA : 8;
   constexpr int H8 = (RA + 7) / 8;
   if (a.conj_in) {
 #pragma unroll
     for (int j = 0; j < RA; ++j) x[j].y = -x[j].y;
+  }
+  if (chirp_in) {
+#pragma unroll
+    for (int j = 0; j < RA; ++j) {
+      const uint32_t e = i + static_cast<uint32_t>(t + RB * j) * tcnt;
+      if (active && e < nlog) x[j] = cmul(x[j], ldt(chirp, e));
+    }
   }
   // inter-pass twiddles: element e = t + RB j times w^(e kmul) =
   // w^((t + RB j0) kmul) w^(8 RB jh kmul), j = j0 + 8 jh, from the two-level table
@@ -301,13 +314,20 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
     GlbC<T> *dst = gc<T>(a.out) + static_cast<int64_t>(bq) * a.bs_out;
     const uint32_t o0 = (i - k) * static_cast<uint32_t>(R) + k;
     const uint32_t pp = static_cast<uint32_t>(a.p);
+    const GlbC<T> *bhat = gc<T>(a.bhat);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
       const int k1 = t + RB * m;
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
-        const uint32_t e2 = static_cast<uint32_t>(k1 + RA * q);
-        stt(dst, o0 + e2 * pp, finish(y[m][q]));
+        const uint32_t o = o0 + static_cast<uint32_t>(k1 + RA * q) * pp;
+        if (a.dst_mode & kDstPointwiseFlag) {
+          stt(dst, o, cconj(cmul(y[m][q], ldt(bhat, o))));
+        } else if (a.dst_mode & kDstChirpFlag) {
+          if (o < nlog) stt(dst, o, finish(cmul(cconj(y[m][q]), ldt(chirp, o))));
+        } else {
+          stt(dst, o, finish(y[m][q]));
+        }
       }
     }
   }

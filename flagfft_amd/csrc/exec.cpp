@@ -744,8 +744,9 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
                         const SeqSet &s, int64_t n, int pass, int npasses) {
   const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
   // register-resident two-stage pass (plain complex chain passes)
+  // (plain complex passes, and Bluestein chain ends: chirp source, pointwise and chirp sinks)
   const bool col_allowed = mode == 0 && opt_.stage_form != 2 && (!opt_.generic_kernels || opt_.queue_form) &&
-                           a.src_mode == kSrcComplex && a.dst_mode == kDstComplex;
+                           (a.src_mode & 3) == kSrcComplex && (a.dst_mode & 3) == kDstComplex;
   const int cid = !col_allowed ? -1
                   : opt_.queue_form ? col_queue_kernel_id(a.R, dbl())
                                     : col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1);
@@ -791,10 +792,12 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
     g.conj_out = a.conj_out;
     g.scale = a.scale;
     g.units = a.units;
-    g.block_store = (rows && p == 1 && a.go.bs == n) ? 1 : 0;
+    g.block_store = (rows && p == 1 && a.go.bs == n && a.dst_mode == kDstComplex) ? 1 : 0;
     g.src_mode = a.src_mode;
     g.dst_mode = a.dst_mode;
     g.nlog = a.nlog;
+    g.chirp = a.chirp;
+    g.bhat = a.bhat;
     const int64_t nblocks = (a.units + cunits - 1) / cunits;
     g.nblocks = static_cast<int32_t>(nblocks);
     g.xcd_remap = opt_.xcd_remap;
@@ -1274,7 +1277,7 @@ void Builder::emit_bluestein(const Core &c, const SeqSet &s, int src, int dst, c
     emit_multipass(*c.conv, sa, src, wP, first, mid, {dst});
     const void *bh = bhat_bluestein(n, M);
     Step &pw = ex_.steps.back();
-    if (pw.kind == Step::Stage)
+    if (pw.kind == Step::Stage || pw.kind == Step::Col)
       pw.sa.bhat = bh;
     else
       pw.pa.bhat = bh;

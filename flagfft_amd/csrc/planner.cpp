@@ -180,6 +180,10 @@ int column_segment_bytes(const std::vector<int> &rad, Prec p) {
 // the run-time-radix pass at about 1.1.
 double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o) {
   std::vector<int> rad;
+  if (o.generic_kernels) {
+    // persistent-queue plans: register-resident queue instances, else the run-time-radix pass
+    return (o.queue_form && col_queue_kernel_id(r, p == Prec::F64) >= 0) ? 1.0 / 4.0 : 1.0 / 1.1;
+  }
   if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0) return first ? 1.0 / 3.8 : 1.0 / 4.6;
   if (radix_factor(r, rad)) {
     const int seg = column_segment_bytes(rad, p);
@@ -337,7 +341,7 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       return c;
     }
     Core sub;
-    if (!multipass_core(M, p, o, sub)) continue;
+    if (!multipass_core(M, p, o, sub, o.queue_form)) continue;
     if (!o.forced_passes.empty()) {
       best = sub;
       have = true;
