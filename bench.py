@@ -562,7 +562,6 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
     inputs = {t: torch.randn(max(need[t] // 4, 1), dtype=torch.float32, device=dev) for t in esz}
     torch.cuda.synchronize()
     tmap = {"C2C": T.C2C, "R2C": T.R2C, "Z2Z": T.Z2Z}
-    out = {}
 
     def run(mode, rate):
         with ff.Server(int(budget_gib * (1 << 30)), max_inflight=3, mode=mode) as srv:
@@ -572,14 +571,23 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
             done = threading.Event()
 
             def reaper():
-                i = 0
-                while i < nreq:
-                    if i < len(tickets):
-                        srv.wait(tickets[i])
-                        srv.release(tickets[i])
-                        i += 1
-                    else:
-                        time.sleep(20e-6)
+                # a client that takes each result as it completes: the oldest
+                # outstanding requests are polled every 100 us and released
+                # when done, in whatever order they finish
+                outstanding, seen, released = [], 0, 0
+                while released < nreq:
+                    while seen < len(tickets):
+                        outstanding.append(tickets[seen])
+                        seen += 1
+                    keep = []
+                    for i, tk in enumerate(outstanding):
+                        if i < 64 and srv.poll(tk)[0]:
+                            srv.release(tk)
+                            released += 1
+                        else:
+                            keep.append(tk)
+                    outstanding = keep
+                    time.sleep(100e-6)
                 done.set()
 
             th = threading.Thread(target=reaper, daemon=True)
@@ -590,6 +598,11 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
             t_next = t0
             for (t, n, b), g in zip(reqs, gaps):
                 t_next += g
+                # sleep (which lets the reaper thread run) until shortly
+                # before the arrival time, then spin to it
+                ahead = t_next - time.perf_counter() - 300e-6
+                if ahead > 0:
+                    time.sleep(ahead)
                 while time.perf_counter() < t_next:
                     pass
                 tickets.append(srv.submit(tmap[t], n, b, inputs[t].data_ptr(), ff.FORWARD))
@@ -597,6 +610,19 @@ def serving_leg(torch, ff, dev, nreq: int, seed: int = 2024, budget_gib: float =
             th.join()
             return srv.stats()
 
+    # the submitting thread and the reaper share the interpreter lock: switch
+    # between them every 100 us (default 5 ms) so that neither holds the other up
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(100e-6)
+    try:
+        return _serving_runs(run, ff, nreq)
+    finally:
+        sys.setswitchinterval(switch)
+
+
+def _serving_runs(run, ff, nreq):
+    """Capacity and Poisson-latency runs of both server modes."""
+    out = {}
     # capacity: every request submitted at once; the plain-launch capacity sets the offered rate
     caps = {name: run(mode, 0.0) for name, mode in (("batched", ff.SERVE_BATCHED), ("plain", ff.SERVE_PLAIN))}
     cap_rate = caps["plain"]["completed"] / caps["plain"]["span_s"]

@@ -80,6 +80,8 @@ bool overrides_to_options(const TuneOverrides &t, PlanOptions &o) {
   if (t.chunk_bytes > 0) o.chunk_bytes = t.chunk_bytes;
   if (t.stage_form < 0 || t.stage_form > 2) return false;
   o.stage_form = t.stage_form;
+  if (t.variant < 0) return false;
+  o.kernel_variant = t.variant;
   if (t.passes.size() > 4 || t.radices.size() > 16) return false;
   o.forced_passes = t.passes;
   o.forced_radices = t.radices;
@@ -675,6 +677,7 @@ bool tuning_from_c(const flagfftxTuning *t, TuneOverrides &o) {
   o.ept = t->ept;
   o.chunk_bytes = t->chunk_bytes;
   o.stage_form = t->stage_form;
+  o.variant = t->variant;
   o.passes.assign(t->passes, t->passes + t->npasses);
   o.radices.assign(t->radices, t->radices + t->nradices);
   return true;

@@ -11,7 +11,6 @@ namespace flagfft {
 namespace {
 // Workgroups of the matrix-core kernel: each of its 4 waves loops over
 // transforms; 1024 groups keep every CU at its register-limited occupancy.
-constexpr int64_t kMfmaGroups = 1024;
 }  // namespace
 
 void hip_check(hipError_t e, const char *what) {
@@ -156,7 +155,7 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       break;
     case Mfma:
       s.ma.nseq = scale(ma.nseq);
-      s.shape.grid = dim3(static_cast<unsigned>(std::min<int64_t>(blocks(s.ma.nseq, 4), kMfmaGroups)));
+      s.shape.grid = dim3(static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks(s.ma.nseq, 4), groups_max))));
       break;
     case BluesteinLds:
     case BluesteinCombine:  // workgroups per sequence are fixed
@@ -209,7 +208,7 @@ void Executable::resolve_kernels() {
       case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Col: s.func = col_kernel(s.leaf_id); break;
-      case Step::Mfma: s.func = mfma_kernel(s.compute); break;
+      case Step::Mfma: s.func = mfma_kernel(s.compute, s.leaf_id); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
       case Step::LeafSplit: s.func = leaf_split_kernel(s.leaf_id, s.dbl, s.leaf_mode); break;
@@ -1030,15 +1029,20 @@ void Builder::emit_matrix_core(const Core &c, const SeqSet &s, int src, int dst,
   a.conj_in = m.conj_in;
   a.conj_out = m.conj_out;
   a.scale = m.scale;
-  // 4 waves per workgroup, each wave loops over transforms; 1024 workgroups
-  // keep every CU at its register-limited occupancy (2 waves per SIMD)
-  const int64_t waves = (s.nbatch + 3) / 4;
-  const int64_t groups = std::max<int64_t>(1, std::min<int64_t>(waves, opt_.forced_units > 0 ? opt_.forced_units : kMfmaGroups));
+  // 4 waves per workgroup (one per SIMD), each wave loops over transforms;
+  // one workgroup per CU, all resident from the start (no workgroup starts
+  // after the first ones finish).  A tuned plan's variant picks the kernel
+  // instance, its units the workgroup count.
+  if (opt_.kernel_variant > mfma_variants()) throw std::invalid_argument("no such matrix-core kernel variant");
+  const int variant = opt_.kernel_variant > 0 ? opt_.kernel_variant - 1 : 0;
+  st.leaf_id = variant;
+  st.groups_max = opt_.forced_units > 0 ? opt_.forced_units : cu_count();
+  const int64_t groups = std::max<int64_t>(1, std::min<int64_t>((s.nbatch + 3) / 4, st.groups_max));
   st.shape = LaunchShape{dim3(static_cast<unsigned>(groups)), dim3(256), 0};
   st.bytes_moved = s.nbatch * 1024 * 8 * 2;
   std::ostringstream lab;
-  lab << "k_mfma1024<" << (c.compute == kComputeFp16x3 ? "true> fp16x3" : "false> bf16x3") << " R=1024 transforms=" << s.nbatch
-      << " groups=" << groups;
+  lab << "k_mfma1024<" << (c.compute == kComputeFp16x3 ? "true," : "false,") << mfma_kernel_targs(variant) << "> "
+      << (c.compute == kComputeFp16x3 ? "fp16x3" : "bf16x3") << " R=1024 transforms=" << s.nbatch << " groups=" << groups;
   st.label = lab.str();
   ex_.steps.push_back(st);
 }

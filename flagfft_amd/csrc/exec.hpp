@@ -72,6 +72,7 @@ struct Step {
   BluesteinCombineArgs ca{};
   LeafSplitArgs lsa{};
   int compute = 0;  // Mfma: kComputeBf16x3 / kComputeFp16x3
+  int64_t groups_max = 0;  // Mfma: workgroups of a launch (resident at once)
   LaunchShape shape{};
   int64_t bytes_moved = 0;  // algorithmic HBM bytes of this launch (read + write)
   std::string label;

@@ -101,7 +101,11 @@ void leaf_split_shape(int id, bool dbl, int &p, int &l, int &ept, std::vector<in
 const void *leaf_split_kernel(int id, bool dbl, int mode);  // mode: LeafMode (real modes: P = 2 instances)
 
 // Split-precision matrix-core transform of length 1024 (kComputeBf16x3 / kComputeFp16x3).
-const void *mfma_kernel(int mode);
+// Split-precision 1024-point kernels: `variant` < mfma_variants() picks the
+// prefetch depth / occupancy instance (0: the default).
+int mfma_variants();
+const void *mfma_kernel(int mode, int variant);
+std::string mfma_kernel_targs(int variant);
 
 // Raise a kernel's dynamic-LDS limit when a launch needs more than 64 KiB.
 hipError_t allow_lds(const void *f, size_t bytes);

@@ -33,6 +33,8 @@
 #include "fft_args.hpp"
 #include "kernels.hpp"
 
+#include <string>
+
 namespace flagfft {
 namespace dev {
 
@@ -95,8 +97,27 @@ __device__ __forceinline__ int out_row(int i) { return 8 * (i >> 3) + 2 * (i & 3
 // w_1024^e from the FP32 table (e reduced mod 1024).
 __device__ __forceinline__ V2<float> w1024(const GlbC<float> *tw, int e) { return ldg(tw + (e & 1023)); }
 
-template <bool F16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_mfma1024(const MfmaArgs a) {
+// Maximum of a non-negative float over the wavefront: DPP exchanges within
+// each row of 16 lanes (quad swaps, half-row and row mirrors), then the four
+// row maxima through scalar registers.  Non-negative floats order as their
+// bit patterns.
+__device__ __forceinline__ float wave_max_nonneg(float m) {
+  int b = __float_as_int(m);
+  b = max(b, __builtin_amdgcn_mov_dpp(b, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  b = max(b, __builtin_amdgcn_mov_dpp(b, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  b = max(b, __builtin_amdgcn_mov_dpp(b, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  b = max(b, __builtin_amdgcn_mov_dpp(b, 0x140, 0xF, 0xF, false));  // row_mirror
+  const int r = max(max(__builtin_amdgcn_readlane(b, 0), __builtin_amdgcn_readlane(b, 16)),
+                    max(__builtin_amdgcn_readlane(b, 32), __builtin_amdgcn_readlane(b, 48)));
+  return __int_as_float(r);
+}
+
+// PF: transforms whose loads are in flight per wave (a ring of PF register
+// sets); WPE: waves per SIMD the register allocation leaves room for.  The
+// launch puts one workgroup on each CU, all resident at once (no tail of
+// workgroups that start when the first ones finish).
+template <bool F16, int PF, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_mfma1024(const MfmaArgs a) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 31;
   const int h = lane >> 5;
@@ -151,9 +172,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const float cout = a.conj_out ? -1.f : 1.f;
   const float user_scale = static_cast<float>(a.scale);
 
-  // software pipeline: the next transform's loads are in flight while this
-  // one runs through the matrix cores (three waves per SIMD fit the registers
-  // with the constant operands in LDS)
+  // software pipeline: the loads of the next PF transforms are in flight
+  // while this one runs through the matrix cores
   auto load = [&](int64_t sq, V2<float> (&x)[2][8]) __attribute__((always_inline)) {
     const GlbC<float> *src = gc<float>(a.in) + sq * a.idist + c;
 #pragma unroll
@@ -161,16 +181,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
       for (int v = 0; v < 8; ++v) x[q][v] = ldg(src + 32 * (16 * q + 2 * v + h));
   };
-  V2<float> xn[2][8];
-  if (wave < a.nseq) load(wave, xn);
-  for (int64_t seq = wave; seq < a.nseq; seq += nwaves) {
+  V2<float> ring[PF][2][8];
+#pragma unroll
+  for (int s = 0; s < PF; ++s)
+    if (wave + s * nwaves < a.nseq) load(wave + s * nwaves, ring[s]);
+  for (int64_t base = wave; base < a.nseq; base += PF * nwaves) {
+#pragma unroll
+  for (int s = 0; s < PF; ++s) {
+    const int64_t seq = base + s * nwaves;
+    if (seq >= a.nseq) break;
     GlbC<float> *dst = gc<float>(a.out) + seq * a.odist + c;
     V2<float> x[2][8];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int v = 0; v < 8; ++v) x[q][v] = xn[q][v];
-    if (seq + nwaves < a.nseq) load(seq + nwaves, xn);
+      for (int v = 0; v < 8; ++v) x[q][v] = ring[s][q][v];
+    if (seq + PF * nwaves < a.nseq) load(seq + PF * nwaves, ring[s]);
     asm volatile("" ::: "memory");  // the constants are read from LDS in every iteration
 
     // exact power-of-two range scaling (FP16 only)
@@ -181,8 +207,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int v = 0; v < 8; ++v) m = fmaxf(m, fmaxf(fabsf(x[q][v].x), fabsf(x[q][v].y)));
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      m = wave_max_nonneg(m);
       if (m > 0.f && m == m && m <= 3.0e38f) {
         int e = 0;
         (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
@@ -242,14 +267,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       stg(dst + 32 * k2, V2<float>{cr[r] * so, cout * ci[r] * so});
     }
   }
+  }
 }
 
 }  // namespace dev
 
-const void *mfma_kernel(int mode) {
-  if (mode == kComputeFp16x3) return reinterpret_cast<const void *>(dev::k_mfma1024<true>);
-  if (mode == kComputeBf16x3) return reinterpret_cast<const void *>(dev::k_mfma1024<false>);
+namespace {
+
+struct MfmaEntry {
+  int pf, wpe;
+  const void *f16, *bf16;
+};
+
+#define FLAGFFT_MFMA(PF, WPE) \
+  {PF, WPE, (const void *)dev::k_mfma1024<true, PF, WPE>, (const void *)dev::k_mfma1024<false, PF, WPE>}
+
+// Variant 0 is the default.  Measured on MI355X at 1024 x 65536 (one
+// workgroup of four waves per CU, the fastest launch for every variant):
+// BF16 191-192 us with PF = 2, FP16 197-200 us with PF = 2 or 3; two or three
+// workgroups per CU ran 5-10 % slower.
+const MfmaEntry kMfmaTable[] = {
+    FLAGFFT_MFMA(2, 2), FLAGFFT_MFMA(3, 2), FLAGFFT_MFMA(1, 3), FLAGFFT_MFMA(2, 1), FLAGFFT_MFMA(4, 1),
+};
+
+}  // namespace
+
+int mfma_variants() { return static_cast<int>(sizeof(kMfmaTable) / sizeof(kMfmaTable[0])); }
+
+const void *mfma_kernel(int mode, int variant) {
+  if (variant < 0 || variant >= mfma_variants()) return nullptr;
+  if (mode == kComputeFp16x3) return kMfmaTable[variant].f16;
+  if (mode == kComputeBf16x3) return kMfmaTable[variant].bf16;
   return nullptr;
+}
+
+std::string mfma_kernel_targs(int variant) {
+  return std::to_string(kMfmaTable[variant].pf) + "," + std::to_string(kMfmaTable[variant].wpe);
 }
 
 }  // namespace flagfft

@@ -34,6 +34,7 @@ struct PlanOptions {
   std::vector<int64_t> forced_passes;  // explicit super-radix split (tuned plans)
   std::vector<int> forced_radices;     // explicit single-pass radix order (tuned plans)
   int forced_units = 0;                // explicit units per workgroup (tuned plans)
+  int kernel_variant = 0;              // 1-based instance of a kernel family with several launch shapes (0: default)
   int stage_form = 0;                  // multi-pass chains: 0 automatic, 1 register-resident passes, 2 LDS super-stages
   int max_threads = 512;               // workgroup size limit of the launches
   bool generic_kernels = false;        // only the run-time-radix kernels (persistent-queue jobs)

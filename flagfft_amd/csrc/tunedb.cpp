@@ -239,7 +239,7 @@ bool tuned_store(const TuneKey &k, const std::string &plan_json, double median_u
 std::string tuning_to_json(const TuneOverrides &t) {
   std::ostringstream o;
   o << "{\"leaf_max\":" << t.leaf_max << ",\"pass_max\":" << t.pass_max << ",\"units\":" << t.units
-    << ",\"ept\":" << t.ept << ",\"chunk_bytes\":" << t.chunk_bytes << ",\"stage_form\":" << t.stage_form
+    << ",\"ept\":" << t.ept << ",\"chunk_bytes\":" << t.chunk_bytes << ",\"stage_form\":" << t.stage_form << ",\"variant\":" << t.variant
     << ",\"passes\":[";
   for (size_t i = 0; i < t.passes.size(); ++i) o << (i ? "," : "") << t.passes[i];
   o << "],\"radices\":[";
@@ -296,6 +296,8 @@ bool tuning_from_json(const std::string &js, TuneOverrides &t) {
   t.chunk_bytes = static_cast<int>(v);
   if (!num("stage_form", v = 0)) return false;
   t.stage_form = static_cast<int>(v);
+  if (!num("variant", v = 0)) return false;
+  t.variant = static_cast<int>(v);
   if (!arr("passes", a)) return false;
   t.passes = a;
   if (!arr("radices", a)) return false;

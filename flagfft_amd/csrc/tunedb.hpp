@@ -23,7 +23,7 @@ struct TuneKey {
 // Plan-time overrides a tuned row holds (flagfftxTuning without the C arrays).
 struct TuneOverrides {
   int64_t leaf_max = 0, pass_max = 0;
-  int units = 0, ept = 0, chunk_bytes = 0, stage_form = 0;
+  int units = 0, ept = 0, chunk_bytes = 0, stage_form = 0, variant = 0;
   std::vector<int64_t> passes;
   std::vector<int> radices;
 };

@@ -79,6 +79,7 @@ class Tuning(ctypes.Structure):
         ("leaf_max", ctypes.c_int), ("pass_max", ctypes.c_int), ("units", ctypes.c_int), ("ept", ctypes.c_int),
         ("chunk_bytes", ctypes.c_int), ("npasses", ctypes.c_int), ("passes", ctypes.c_int * 4),
         ("nradices", ctypes.c_int), ("radices", ctypes.c_int * 16), ("stage_form", ctypes.c_int),
+        ("variant", ctypes.c_int),
     ]
 
 
@@ -272,7 +273,7 @@ def plan_stats(h, direction: int = FORWARD):
 
 def make_tuning(**kw) -> Tuning:
     """flagfftxTuning from keyword overrides (leaf_max, pass_max, units, ept, chunk_bytes, passes, radices,
-    stage_form)."""
+    stage_form, variant)."""
     t = Tuning()
     for k, v in kw.items():
         if k == "passes":

@@ -64,6 +64,9 @@ typedef struct flagfftxTuning_t {
   int stage_form;     /* kernel of the passes of a multi-pass chain: 0 automatic,
                          1 register-resident two-stage passes where compiled,
                          2 LDS Stockham super-stages only */
+  int variant;        /* instance of a kernel family compiled in several launch
+                         shapes, 1-based (0: the default): the split-precision
+                         1024-point kernel's prefetch depth / occupancy */
 } flagfftxTuning;
 
 flagfftResult flagfftxSetTuning(const flagfftxTuning *tuning); /* NULL restores defaults */

@@ -152,20 +152,27 @@ def test_full_configuration_1024x65536_all_paths(gpu, direction):
         assert worst == worst and worst <= bound[compute] + bound[ff.COMPUTE_VECTOR]
 
 
+# (prefetch depth, waves per SIMD) of the kernel variants, tuning variant = index + 1
+MFMA_VARIANTS = [(2, 2), (3, 2), (1, 3), (2, 1), (4, 1)]
+
+
 @pytest.mark.parametrize("compute", [ff.COMPUTE_FP16X3, ff.COMPUTE_BF16X3], ids=["fp16x3", "bf16x3"])
 @pytest.mark.parametrize("direction", [ff.FORWARD, ff.INVERSE], ids=["fwd", "inv"])
-def test_wave_loop_and_prefetch_with_few_groups(gpu, compute, direction):
-    """Two workgroups (8 waves) for 200 transforms: every wave runs its loop
-    25 times, each iteration on the transform its previous one prefetched."""
-    batch = 200
+@pytest.mark.parametrize("variant", range(1, len(MFMA_VARIANTS) + 1), ids=[f"pf{p}_wpe{w}" for p, w in MFMA_VARIANTS])
+def test_wave_loop_and_prefetch_with_few_groups(gpu, compute, direction, variant):
+    """Two workgroups (8 waves) for 203 transforms: every wave runs its loop
+    25 or 26 times, each iteration on a transform an earlier one prefetched
+    (a ring of PF transforms in flight; 203 leaves a partial last round)."""
+    batch = 203
     x = complex_input(ora.accuracy_seed(int(ff.Type.C2C), N, batch, 10), (batch, N), np.complex64, "normal")
-    ff.check("flagfftxSetTuning", ff.set_tuning(units=2))
+    ff.check("flagfftxSetTuning", ff.set_tuning(units=2, variant=variant))
     try:
         y, desc = _run(gpu, x, direction, compute)
     finally:
         ff.set_tuning()
-    assert "groups=2" in desc, desc
+    pf, wpe = MFMA_VARIANTS[variant - 1]
+    assert "groups=2" in desc and f"{pf},{wpe}>" in desc, desc
     ref = ora.fft(x.astype(np.complex128), sign=direction)
     l2, _ = ora.rel_l2(y, ref, N)
-    print(f"{NAMES[compute]} 2 groups x 4 waves, batch {batch}: rel_l2={l2:.3e}")
+    print(f"{NAMES[compute]} PF={pf} WPE={wpe}, 2 groups x 4 waves, batch {batch}: rel_l2={l2:.3e}")
     assert np.isfinite(l2) and l2 <= BOUND[compute]
