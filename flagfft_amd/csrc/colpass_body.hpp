@@ -63,12 +63,33 @@ __device__ __forceinline__ void pow8(V2<T> w, V2<T> *p) {
   p[7] = cmul(p[4], p[3]);
 }
 
+// Loads of tile `blk`: element e = t + RB j of the thread's unit (zero
+// outside the units and, for a Bluestein chirp source, past the user length).
+template <typename T, int C, int RA, int RB>
+__device__ __forceinline__ void col_load(const StageArgs &a, int blk, V2<T> (&x)[RA]) {
+  const int tid = threadIdx.x;
+  const int sl = tid % C;
+  const int t = tid / C;
+  const int64_t u0 = static_cast<int64_t>(blk) * C;
+  const bool active = u0 + sl < a.units;
+  const uint32_t bq = fdiv(static_cast<uint32_t>(u0), a.fd_tcnt);
+  const uint32_t i = static_cast<uint32_t>(u0) - bq * static_cast<uint32_t>(a.tcnt) + sl;
+  const uint32_t tcnt = static_cast<uint32_t>(a.tcnt);
+  const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in;
+  const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
+  const uint32_t nlog = static_cast<uint32_t>(a.nlog);
+#pragma unroll
+  for (int j = 0; j < RA; ++j) {
+    const uint32_t e = i + static_cast<uint32_t>(t + RB * j) * tcnt;
+    x[j] = (active && (!chirp_in || e < nlog)) ? ldt(src, e) : V2<T>{0, 0};
+  }
+}
+
+// Everything after the loads of tile `blk` (x: what col_load gave).
 // TWM: twiddles from table loads per power (0) or from two or three loads and
 // products (1: w^j = w^(j mod 8) (w^8)^(j / 8), fewer memory instructions).
-// One tile (workgroup `blk` of the launch) of a register-resident pass; every
-// one of the C * RB threads of the workgroup must call it (it synchronises).
 template <typename T, int C, int RA, int RB, int NCH, int TWM>
-__device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
+__device__ __forceinline__ void col_rest(const StageArgs &a, int blk, V2<T> (&x)[RA]) {
   constexpr int R = RA * RB;
   constexpr int NM = RA / RB;   // stage-B transforms per thread
   constexpr int KC = RA / NCH;  // k1 values per exchange chunk
@@ -92,7 +113,6 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
   const uint32_t k = i - fdiv(i, a.fd_p) * static_cast<uint32_t>(a.p);
   const uint32_t kt = first ? 0u : fdiv(k, a.fd_ncol);
   const uint32_t tcnt = static_cast<uint32_t>(a.tcnt);
-  const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in;
   const GlbC<T> *tws = gc<T>(a.tw_small);
   const GlbC<T> *tw_lo = gc<T>(a.tw_lo);
   const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
@@ -100,19 +120,13 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
   const uint32_t kmul = kt * static_cast<uint32_t>(a.twp_mult);
   const T scale = static_cast<T>(a.scale);
 
-  // ---- loads: element e = t + RB j of the unit --------------------------------
+  // ---- input modes of the loaded elements e = t + RB j of the unit -----------
   // Bluestein chains: the first pass reads chirp[e] x[e] for e < n (zero past
   // the user length n = nlog), the last passes write conj(X bhat) (forward
   // chain) or conj(X) chirp for e < n (inverse chain)
   const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
   const uint32_t nlog = static_cast<uint32_t>(a.nlog);
   const GlbC<T> *chirp = gc<T>(a.chirp);
-  V2<T> x[RA];
-#pragma unroll
-  for (int j = 0; j < RA; ++j) {
-    const uint32_t e = i + static_cast<uint32_t>(t + RB * j) * tcnt;
-    x[j] = (active && (!chirp_in || e < nlog)) ? ldt(src, e) : V2<T>{0, 0};
-  }
   constexpr int L8 = RA < 8 ? RA : 8;
   constexpr int H8 = (RA + 7) / 8;
   if (a.conj_in) {
@@ -203,15 +217,23 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
     const int rsl = bs ? tid / RA : sl;  // unit slot of this thread's stage-B transform
     const int rk1 = bs ? tid % RA : t;   // and its k1
     const int hw = rk1 / H;
+    // odd t write their upper k1 half in chunk 0: swap the halves in
+    // registers first (constant indices: a runtime index into x would put the
+    // array in scratch memory)
+    if (t & 1) {
+#pragma unroll
+      for (int kk = 0; kk < H; ++kk) {
+        const V2<T> tmp = x[kk];
+        x[kk] = x[H + kk];
+        x[H + kk] = tmp;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       if (c > 0) __syncthreads();  // the previous chunk's reads are done
       const int h = (t & 1) ^ c;    // the k1 half this thread writes in chunk c
 #pragma unroll
-      for (int kk = 0; kk < H; ++kk) {
-        const V2<T> v = h ? x[H + kk] : x[kk];
-        stc(lds + (h * H + kk) * pitch + (t >> 1) * C + sl, v);
-      }
+      for (int kk = 0; kk < H; ++kk) stc(lds + (h * H + kk) * pitch + (t >> 1) * C + sl, x[c * H + kk]);
       __syncthreads();
       const int par = hw ^ c;  // tt parity this thread reads in chunk c
 #pragma unroll
@@ -331,6 +353,15 @@ __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
       }
     }
   }
+}
+
+// One tile (workgroup `blk` of the launch) of a register-resident pass; every
+// one of the C * RB threads of the workgroup must call it (it synchronises).
+template <typename T, int C, int RA, int RB, int NCH, int TWM>
+__device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
+  V2<T> x[RA];
+  col_load<T, C, RA, RB>(a, blk, x);
+  col_rest<T, C, RA, RB, NCH, TWM>(a, blk, x);
 }
 
 }  // namespace dev
