@@ -38,16 +38,18 @@ struct ColEntry {
 
 // Measured on MI355X (2^20 x 16 and 2^24 C2C FP32): for R = 1024 the first
 // pass (contiguous stores) runs best with 32 units per workgroup, later
-// passes (C-element stores) with 16; for R = 256, 16 units throughout.
+// passes (C-element stores) with 16; for R = 256 the first pass with 16
+// units (65-67 us per 2^24 pass against 72-73 with 64), later passes with 64
+// units in two exchange chunks (55-59 us against 62-65 with 16).
 const ColEntry kColTable[] = {
     FLAGFFT_COL(float, 16, 32, 32, 2, 4, 2),          // 1024: 128-byte runs, 68 KB LDS, two workgroups per CU
     FLAGFFT_COL(float, 32, 32, 32, 2, 4, 1),          // 1024: 256-byte runs, 136 KB LDS, 1024 threads
     FLAGFFT_COL(float, 16, 32, 32, 1, 2, 0),          // 1024: 132 KB LDS, one workgroup per CU
     FLAGFFT_COLQ(float, 8, 32, 32, 2, 4, 0, 0),       // 1024: 64-byte runs, 35 KB LDS, four workgroups per CU
     FLAGFFT_COL(float, 8, 32, 32, 1, 2, 0),           // 1024: 64-byte runs
-    FLAGFFT_COLQ(float, 16, 16, 16, 1, 2, 3, 2),      // 256: 128-byte runs, 33 KB LDS
+    FLAGFFT_COLQ(float, 16, 16, 16, 1, 2, 1, 2),      // 256: 128-byte runs, 33 KB LDS
     FLAGFFT_COL(float, 32, 16, 16, 1, 4, 0),          // 256: 256-byte runs, 66 KB LDS
-    FLAGFFT_COL(float, 64, 16, 16, 2, 4, 0),          // 256: 512-byte runs, one workgroup per CU
+    FLAGFFT_COL(float, 64, 16, 16, 2, 4, 2),          // 256: 512-byte runs, 66 KB LDS, 1024 threads
     FLAGFFT_COLQ(float, 16, 32, 16, 1, 2, 3, 1),      // 512: 128-byte runs
     FLAGFFT_COLQ(float, 32, 16, 8, 1, 2, 3, 3),       // 128
     FLAGFFT_COL(float, 64, 8, 8, 1, 4, 3),            // 64: 512-byte runs
@@ -78,6 +80,18 @@ int col_kernel_id(int64_t r, bool dbl, int units, bool first) {
 }
 
 int col_kernel_id(int64_t r, bool dbl, int units) { return col_kernel_id(r, dbl, units, false); }
+
+int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt) {
+  const int n = static_cast<int>(sizeof(kColTable) / sizeof(kColTable[0]));
+  int any = -1;
+  for (int id = 0; id < n; ++id) {
+    const ColEntry &e = kColTable[id];
+    if (e.r != r || e.dbl != dbl || tcnt % e.c != 0) continue;
+    if (e.pref != 0) return id;
+    if (any < 0) any = id;
+  }
+  return any;
+}
 
 int col_queue_kernel_id(int64_t r, bool dbl) {
   const int n = static_cast<int>(sizeof(kColTable) / sizeof(kColTable[0]));

@@ -742,13 +742,13 @@ Builder::TwoLevel Builder::two_level(int64_t n) {
 bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int mode, int src, int dst,
                         const SeqSet &s, int64_t n, int pass, int npasses) {
   const int sid = opt_.generic_kernels ? -1 : stage_kernel_id(radices, dbl(), opt_.forced_units);
-  // register-resident two-stage pass (plain complex chain passes)
-  // (plain complex passes, and Bluestein chain ends: chirp source, pointwise and chirp sinks)
+  // register-resident two-stage pass (plain complex passes, and Bluestein
+  // chain ends: chirp source, pointwise and chirp sinks)
   const bool col_allowed = mode == 0 && opt_.stage_form != 2 && (!opt_.generic_kernels || opt_.queue_form) &&
                            (a.src_mode & 3) == kSrcComplex && (a.dst_mode & 3) == kDstComplex;
-  const int cid = !col_allowed ? -1
-                  : opt_.queue_form ? col_queue_kernel_id(a.R, dbl())
-                                    : col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1);
+  int cid = !col_allowed        ? -1
+            : opt_.queue_form ? col_queue_kernel_id(a.R, dbl())
+                              : col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1);
   if ((sid < 0 && cid < 0) || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
   // contiguous sequences, or the columns of contiguous rows of ncol elements
   const bool rows = !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1;
@@ -762,6 +762,12 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   int cunits = 0, cthreads = 0, cebytes = 0;
   size_t clds = 0;
   if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
+  // the default instance's tile does not divide the units of a sequence:
+  // another instance of the radix whose tile does
+  if (cid >= 0 && tcnt % cunits != 0 && opt_.forced_units == 0 && !opt_.queue_form) {
+    cid = col_kernel_id_dividing(R, dbl(), tcnt);
+    if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
+  }
   // the register-resident pass addresses a sequence from a uniform base with
   // 32-bit offsets: its tiles may not straddle sequences, nor a sequence 4 GiB
   const bool col_ok = cid >= 0 && tcnt % cunits == 0 && tcnt * R * eb() < (int64_t{1} << 32);

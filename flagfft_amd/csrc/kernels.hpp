@@ -51,6 +51,9 @@ const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 
 // its units, threads, dynamic LDS bytes and element bytes; the kernel.
 int col_kernel_id(int64_t r, bool dbl, int units);
 int col_kernel_id(int64_t r, bool dbl, int units, bool first);
+// An instance of radix r whose units per workgroup divide tcnt (a default
+// instance first), or -1.
+int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt);
 void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes);
 const void *col_kernel(int id);
 // The instance of 256 threads the persistent queue kernel also runs (-1 if
