@@ -20,7 +20,8 @@ LIB = os.path.join(PKG, "libflagfft.so")
 ARCH = os.environ.get("FLAGFFT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["kernels.hip", "leaf.hip", "stage.hip", "colpass.hip", "bluestein_lds.hip", "leaf_split.hip", "mfma.hip", "planner.cpp", "exec.cpp", "api.cpp", "serve.cpp", "tunedb.cpp"]
-HEADERS = ["fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp", "planner.hpp", "exec.hpp", "tunedb.hpp"]
+HEADERS = ["fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp", "colpass_body.hpp",
+           "planner.hpp", "exec.hpp", "tunedb.hpp"]
 
 COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -35,7 +36,7 @@ COMMON_FLAGS = [
 # the planner and kernels it was measured with.
 PLANNER_SOURCES = ["planner.cpp", "planner.hpp", "exec.cpp", "exec.hpp"]
 KERNEL_SOURCES = [s for s in SOURCES if s.endswith(".hip")] + [
-    "fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp"]
+    "fft_args.hpp", "codelets.hpp", "dev_common.hpp", "spec_common.hpp", "kernels.hpp", "colpass_body.hpp"]
 
 
 def source_hash(names) -> str:
