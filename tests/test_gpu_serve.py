@@ -303,3 +303,46 @@ def test_multi_pass_requests_run_in_queue_launches(gpu):
         assert np.array_equal(a, m), f"{t.name} n={n}: result differs between alone and in the mix"
         l2, _ = ora.rel_l2(m, _oracle(t, n, x, d), m.shape[-1])
         assert l2 <= _bound(t, n), f"{t.name} n={n} rel_l2={l2:.3e}"
+
+
+def test_small_requests_pass_one_that_does_not_fit_yet(gpu):
+    """BATCHED admission: while a request waits for arena memory (an earlier
+    result is still held by the client), smaller requests behind it are
+    admitted and complete; the waiting one runs once the memory is released."""
+    import time
+
+    n, b = 65536, 16  # two-pass chain: 8 MiB of output and 8 MiB of workspace
+    big = complex_input(901, (b, n), np.complex64, "normal")
+    small = [complex_input(902 + i, (5, 64), np.complex64, "normal") for i in range(4)]
+    big_d = dev_tensor(big, gpu)
+    small_d = [dev_tensor(x, gpu) for x in small]
+    sync(gpu)
+
+    def poll_until(srv, tk, seconds):
+        t0 = time.time()
+        while time.time() - t0 < seconds:
+            if srv.poll(tk)[0]:
+                return True
+            time.sleep(1e-3)
+        return False
+
+    with ff.Server(20 << 20, max_inflight=2) as srv:
+        holder = srv.submit(T.C2C, n, b, big_d.data_ptr(), ff.FORWARD)
+        assert poll_until(srv, holder, 30.0)  # done; its output stays held
+        waiting = srv.submit(T.C2C, n, b, big_d.data_ptr(), ff.FORWARD)  # 8 + 16 MiB > 20 MiB
+        smalls = [srv.submit(T.C2C, 64, 5, x.data_ptr(), ff.FORWARD) for x in small_d]
+        for tk, x in zip(smalls, small):
+            assert poll_until(srv, tk, 30.0), "a small request was held behind one that does not fit"
+            ptr, _ = srv.wait(tk)
+            y = d2h(ptr, (5, 64), np.complex64)
+            srv.release(tk)
+            l2, _ = ora.rel_l2(y, ora.fft(x.astype(np.complex128)), 64)
+            assert l2 <= _bound(T.C2C, 64)
+        assert not srv.poll(waiting)[0], "the waiting request ran without its memory"
+        srv.release(holder)
+        ptr, _ = srv.wait(waiting)
+        y = d2h(ptr, (b, n), np.complex64)
+        srv.release(waiting)
+    rows = [0, b - 1]
+    l2, _ = ora.rel_l2(y[rows], ora.fft(big[rows].astype(np.complex128)), n)
+    assert l2 <= _bound(T.C2C, n)
