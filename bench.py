@@ -696,6 +696,10 @@ def accuracy(case, torch):
 def run_workload(spec, args, torch, ff, dev, stream, seed):
     # one stream per transform family for the throughput region (default priority)
     cases = [Case(*row[:5], dev, ff, torch, *row[5:]) for row in spec]
+    if args.graph:
+        for c in cases:
+            for p in c.plans:
+                p.set_graph_mode(True)
     footprint = sum(c.footprint() for c in cases)
     nsets = max(1, -(-2 * MALL_BYTES // max(footprint, 1)) + 1)
     nsets = min(nsets, 64)
@@ -741,6 +745,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64-oracle error reports")
     ap.add_argument("--headline-only", action="store_true", help="measure the headline workload only")
+    ap.add_argument("--graph", action="store_true", help="run every plan's launches as one hipGraph (flagfftxSetGraphMode)")
     ap.add_argument("--suite", action="store_true", help="also measure further configurations (stderr)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving leg (0: skip)")
     ap.add_argument("--streams", type=int, default=0,
