@@ -36,12 +36,17 @@ struct ColEntry {
 #define FLAGFFT_COLQ(T, C, RA, RB, NCH, WPE, PREF, QVAR) \
   {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1>, PREF, QVAR}
 
-// Measured on MI355X (2^20 x 16 and 2^24 C2C FP32): for R = 1024 the first
+// Measured on MI355X (2^20 x 16 and 2^24 C2C FP32): for R = 4096 (64 x 64
+// in registers, 198 VGPRs, one 512-thread workgroup per CU) 8 units per
+// workgroup, which makes 2^24 two passes instead of three (150 against
+// 185 us) and the 4096-point columns of a 4096^2 transform one pass instead
+// of two (75 against 95 us); for R = 1024 the first
 // pass (contiguous stores) runs best with 32 units per workgroup, later
 // passes (C-element stores) with 16; for R = 256 the first pass with 16
 // units (65-67 us per 2^24 pass against 72-73 with 64), later passes with 64
 // units in two exchange chunks (55-59 us against 62-65 with 16).
 const ColEntry kColTable[] = {
+    FLAGFFT_COL(float, 8, 64, 64, 2, 2, 3),           // 4096: 64-byte runs, 135 KB LDS, 198 VGPRs
     FLAGFFT_COL(float, 16, 32, 32, 2, 4, 2),          // 1024: 128-byte runs, 68 KB LDS, two workgroups per CU
     FLAGFFT_COL(float, 32, 32, 32, 2, 4, 1),          // 1024: 256-byte runs, 136 KB LDS, 1024 threads
     FLAGFFT_COL(float, 16, 32, 32, 1, 2, 0),          // 1024: 132 KB LDS, one workgroup per CU
@@ -80,6 +85,11 @@ int col_kernel_id(int64_t r, bool dbl, int units, bool first) {
 }
 
 int col_kernel_id(int64_t r, bool dbl, int units) { return col_kernel_id(r, dbl, units, false); }
+
+int col_run_bytes(int64_t r, bool dbl, bool first) {
+  const int id = col_kernel_id(r, dbl, 0, first);
+  return id < 0 ? 0 : kColTable[id].c * (dbl ? 16 : 8);
+}
 
 int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt) {
   const int n = static_cast<int>(sizeof(kColTable) / sizeof(kColTable[0]));

@@ -136,6 +136,8 @@ Step Step::rebatched(int64_t batch, int64_t built) const {
       int per = 1, thr = 0, ebytes = 0;
       size_t lds = 0;
       col_kernel_shape(leaf_id, per, thr, lds, ebytes);
+      // a tile may end partially only when the pass has a single sequence
+      if (sa.tcnt % per != 0) throw std::logic_error("re-batched pass whose tiles would straddle sequences");
       s.sa.units = scale(sa.units);
       s.sa.nblocks = static_cast<int32_t>(blocks(s.sa.units, per));
       s.shape.grid = dim3(static_cast<unsigned>(s.sa.nblocks));
@@ -764,13 +766,14 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
   // the default instance's tile does not divide the units of a sequence:
   // another instance of the radix whose tile does
-  if (cid >= 0 && tcnt % cunits != 0 && opt_.forced_units == 0 && !opt_.queue_form) {
+  if (cid >= 0 && tcnt % cunits != 0 && a.units != tcnt && opt_.forced_units == 0 && !opt_.queue_form) {
     cid = col_kernel_id_dividing(R, dbl(), tcnt);
     if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
   }
   // the register-resident pass addresses a sequence from a uniform base with
   // 32-bit offsets: its tiles may not straddle sequences, nor a sequence 4 GiB
-  const bool col_ok = cid >= 0 && tcnt % cunits == 0 && tcnt * R * eb() < (int64_t{1} << 32);
+  // (one sequence: the last tile may be partial, its lanes past the units idle)
+  const bool col_ok = cid >= 0 && (tcnt % cunits == 0 || a.units == tcnt) && tcnt * R * eb() < (int64_t{1} << 32);
   if (!col_ok && sid < 0) return false;
   if (col_ok) {
     Step cst;

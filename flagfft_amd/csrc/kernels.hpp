@@ -51,6 +51,8 @@ const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 
 // its units, threads, dynamic LDS bytes and element bytes; the kernel.
 int col_kernel_id(int64_t r, bool dbl, int units);
 int col_kernel_id(int64_t r, bool dbl, int units, bool first);
+// Row-segment bytes (units x element bytes) of the default instance, 0 if none.
+int col_run_bytes(int64_t r, bool dbl, bool first);
 // An instance of radix r whose units per workgroup divide tcnt (a default
 // instance first), or -1.
 int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt);

@@ -175,7 +175,8 @@ int column_segment_bytes(const std::vector<int> &rad, Prec p) {
 
 // Relative time of one HBM sweep by a pass of super-radix r: the inverse of
 // its measured rate on MI355X (TB/s of read + write bytes).  Register-resident
-// two-stage passes run at 3.8 (first pass, from uncached input) to 4.6;
+// two-stage passes run at 3.8 (first pass, from uncached input) to 4.6 with
+// row segments of 128 bytes or more, 3.0 to 4.2 with 64;
 // LDS super-stages at 2.2 to 3.5 by the row-segment bytes their tile reads;
 // the run-time-radix pass at about 1.1.
 double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o) {
@@ -184,7 +185,12 @@ double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o) {
     // persistent-queue plans: register-resident queue instances, else the run-time-radix pass
     return (o.queue_form && col_queue_kernel_id(r, p == Prec::F64) >= 0) ? 1.0 / 4.0 : 1.0 / 1.1;
   }
-  if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0) return first ? 1.0 / 3.8 : 1.0 / 4.6;
+  if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0) {
+    // 64-byte segments (the 4096-point instance: 8 units of 32 KiB in one
+    // workgroup) run at 3.0 / 4.2
+    if (col_run_bytes(r, p == Prec::F64, first) < 128) return first ? 1.0 / 3.0 : 1.0 / 4.2;
+    return first ? 1.0 / 3.8 : 1.0 / 4.6;
+  }
   if (radix_factor(r, rad)) {
     const int seg = column_segment_bytes(rad, p);
     if (seg >= 128) return 1.0 / 3.5;
@@ -444,9 +450,24 @@ Plan1D plan_1d(int64_t n, bool real_fwd, bool real_inv, Prec p, const PlanOption
 // columns), two passes of wide tiles move the data twice at full segment
 // width and win (4096-point FP32 columns: 66 us against 108 us per
 // 2049 x 4096 half spectrum).
-void widen_columns(Plan1D &cols, Prec prec, const PlanOptions &o) {
+// ncol columns per transform, nseq transforms.
+void widen_columns(Plan1D &cols, Prec prec, const PlanOptions &o, int64_t ncol, int64_t nseq) {
   const bool tuned = !o.forced_passes.empty() || !o.forced_radices.empty() || o.forced_units != 0 || o.generic_kernels;
   if (tuned || cols.core.kind != CoreKind::Leaf) return;
+  // a register-resident instance of the whole column length reads 64-byte
+  // segments or more in one pass (4096-point FP32 columns: 75 us against
+  // 95 us for two 64-point passes), when its workgroups (one per CU) fill the
+  // 256 CUs in whole rounds (2049 half-spectrum columns are 257 tiles: the
+  // last one would run alone, 228 against 173 us for the 4096^2 R2C->C2R
+  // step) and tile whole transforms
+  const int run = o.stage_form != 2 ? col_run_bytes(cols.n, prec == Prec::F64, false) : 0;
+  if (run >= 64) {
+    constexpr int64_t kCus = 256;
+    const int64_t c = run / elem_bytes(prec);
+    const int64_t tiles = (ncol * nseq + c - 1) / c;
+    const int64_t rounds = (tiles + kCus - 1) / kCus;
+    if ((nseq == 1 || ncol % c == 0) && 4 * tiles >= 3 * rounds * kCus) return;
+  }
   const int seg = column_segment_bytes(cols.core.radices, prec);
   Core mc;
   if (seg == 0 || seg >= 64 || !split_passes(cols.n, 2, pass_max(o, prec), mc.passes)) return;
@@ -481,13 +502,15 @@ PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
   } else if (d.rank == 2) {
     t.rows = plan_1d(d.n1, d.real_fwd, d.real_inv, d.prec, o);
     t.cols = plan_1d(d.n0, false, false, d.prec, o);
-    widen_columns(t.cols, d.prec, o);
+    const bool half = d.real_fwd || d.real_inv;
+    widen_columns(t.cols, d.prec, o, half ? d.n1 / 2 + 1 : d.n1, d.batch);
   } else {
     t.rows = plan_1d(d.n2, d.real_fwd, d.real_inv, d.prec, o);
     t.mid = plan_1d(d.n1, false, false, d.prec, o);
     t.cols = plan_1d(d.n0, false, false, d.prec, o);
-    widen_columns(t.mid, d.prec, o);
-    widen_columns(t.cols, d.prec, o);
+    const int64_t n2c = (d.real_fwd || d.real_inv) ? d.n2 / 2 + 1 : d.n2;
+    widen_columns(t.mid, d.prec, o, n2c, d.n0 * d.batch);
+    widen_columns(t.cols, d.prec, o, d.n1 * n2c, d.batch);
   }
   return t;
 }

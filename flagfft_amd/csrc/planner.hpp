@@ -105,6 +105,7 @@ int leaf_kernel_id(const std::vector<int> &radices);
 // Register-resident two-stage pass instance for super-radix r (colpass.hip), -1 if none.
 int col_kernel_id(int64_t r, bool dbl, int units);
 int col_queue_kernel_id(int64_t r, bool dbl);
+int col_run_bytes(int64_t r, bool dbl, bool first);
 // True when every super-radix of the split has a specialised kernel.
 bool passes_specialised(const std::vector<int64_t> &passes);
 

@@ -249,9 +249,15 @@ def test_planner_choices(untuned):
     assert "RealEvenFused(n=6000" in _describe(6000, T.R2C)
     assert "RealEvenSplitPass(n=32768" in _describe(32768, T.C2R)
     assert "RealPromoted(n=243" in _describe(243, T.D2Z)
-    assert "super_radices=[256,256,256]" in _describe(1 << 24)   # register-resident passes beat [4096,4096]
+    assert "super_radices=[4096,4096]" in _describe(1 << 24)     # two register-resident 64 x 64 passes
+    assert "super_radices=[256,256,256]" in _describe(1 << 24, T.Z2Z)  # FP64: no 4096-point instance
     assert "super_radices=[1024,1024]" in _describe(1 << 20)
     assert "TwoDim(n0=4096, n1=4096" in _describe([4096, 4096], T.R2C, rank=2)
+    # 4096-point columns: one register-resident pass when its 8-column tiles
+    # fill the CUs in whole rounds (4096 columns), two 64-point passes for
+    # the 2049 half-spectrum columns
+    assert "super_radices=[64,64]" not in _describe([4096, 4096], T.C2C, rank=2)
+    assert "super_radices=[64,64]" in _describe([4096, 4096], T.R2C, rank=2)
 
 
 def test_tuning_overrides_reach_the_planner(untuned):

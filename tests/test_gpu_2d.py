@@ -178,3 +178,54 @@ def test_config_2d(gpu):
         p.z2z(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
         sync(gpu)
     check_accuracy(host(yd), ora.fft2(x), ff.Type.Z2Z, n0 * n1, row_len=n0 * n1, what="2048^2 Z2Z")
+
+
+# 4096-point columns run as one register-resident pass of 8 columns per
+# workgroup (64 x 64 points in registers) when the tiles fill the CUs in
+# whole rounds; with one transform the last tile may be partial (ncol not a
+# multiple of 8: its idle lanes must neither read nor write), with several
+# each transform needs whole tiles.  Other column counts take two 64-point
+# passes.
+@pytest.mark.parametrize("shape,batch,single", [((4096, 4093), 1, True), ((4096, 2045), 1, True),
+                                                ((4096, 1024), 2, True), ((4096, 333), 1, False)],
+                         ids=["4096x4093", "4096x2045", "4096x1024-b2", "4096x333"])
+def test_4096_point_columns(gpu, shape, batch, single):
+    n0, n1 = shape
+    x = complex_input(n0 + 3 * n1 + batch, (batch, n0, n1), np.complex64)
+    xd = dev_tensor(x, gpu)
+    yd = xd.clone().zero_()
+    with _plan(n0, n1, ff.Type.C2C, batch) as p:
+        desc = p.description()
+        p.set_stream(stream_of(gpu))
+        p.c2c(xd.data_ptr(), yd.data_ptr(), ff.FORWARD)
+        sync(gpu)
+    assert ("k_col<fp32,8,64,64" in desc) == single, desc
+    ref = ora.fft2(x.astype(np.complex128))
+    n = n0 * n1
+    check_accuracy(host(yd).reshape(batch, n), ref.reshape(batch, n), ff.Type.C2C, n, row_len=n, factor=2.0,
+                   what=f"2D {n0}x{n1} batch {batch}")
+
+
+def test_4096_point_columns_of_real_halves(gpu):
+    """R2C / C2R 4096 x 4096: 2049 half-spectrum columns would leave one tile
+    of the single-pass form alone on the GPU; they take two 64-point passes."""
+    n0 = n1 = 4096
+    x = real_input(0x4096, (1, n0, n1), np.float32, kind="normal")
+    xd = dev_tensor(x, gpu)
+    h = n1 // 2 + 1
+    import torch
+    yd = torch.zeros((1, n0, h), dtype=torch.complex64, device=gpu)
+    zd = torch.zeros_like(xd)
+    with ff.Plan.two_d(n0, n1, ff.Type.R2C) as pf, ff.Plan.two_d(n0, n1, ff.Type.C2R) as pi:
+        assert "super_radices=[64,64]" in pf.description()
+        for p in (pf, pi):
+            p.set_stream(stream_of(gpu))
+        pf.r2c(xd.data_ptr(), yd.data_ptr())
+        pi.c2r(yd.data_ptr(), zd.data_ptr())
+        sync(gpu)
+    n = n0 * n1
+    ref = ora.rfft2(x.astype(np.float64))
+    check_accuracy(host(yd).reshape(1, -1), ref.reshape(1, -1), ff.Type.R2C, n, row_len=n0 * h, factor=2.0,
+                   what="2D R2C 4096x4096")
+    check_accuracy(host(zd).reshape(1, -1), n * x.astype(np.float64).reshape(1, -1), ff.Type.C2R, n, row_len=n,
+                   factor=4.0, what="2D C2R roundtrip 4096x4096")
