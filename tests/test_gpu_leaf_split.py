@@ -2,10 +2,10 @@
 
 k_leaf_split computes X[P k + h] = FFT_L(y_h)[k] with
 y_h[n1] = w_N^(n1 h) sum_n2 x[n1 + L n2] w_P^(n2 h) (N = P L) on workgroup
-(sequence, h).  The planner takes it for contiguous complex batches smaller
-than half the CU count when the length has an instance (8192, 16384 and
-10935 = 5 x 3^7), in place of one workgroup per sequence or a multi-pass
-chain.  Checked against the FP64 oracle within the stated bounds, both
+(sequence, h).  The planner takes it for contiguous complex batches whose
+split workgroups (batch x P) are at most twice the CU count when the length
+has an instance (8192, 16384 and 10935 = 5 x 3^7), in place of one
+workgroup per sequence or a multi-pass chain.  Checked against the FP64 oracle within the stated bounds, both
 directions, in place, with a sequence count that is and is not a multiple of
 8 (the XCD-aligned and the plain workgroup order), and against the non-split
 plan of the same length.
