@@ -201,7 +201,7 @@ class Server {
     // with register-resident passes (MiB of input + output; default 64)
     if (const char *e = std::getenv("FLAGFFT_SERVE_QUEUE_MAX_MIB")) {
       const long long mib = std::atoll(e);
-      if (mib >= 0) queue_max_fast_ = static_cast<int64_t>(mib) << 20;
+      if (mib >= 0) queue_max_fast_ = static_cast<int64_t>(std::min<long long>(mib, 1LL << 30)) << 20;
     }
     hip_check(hipGetDevice(&device_), "hipGetDevice");
     hipDeviceProp_t prop;
