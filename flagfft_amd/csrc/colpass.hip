@@ -14,9 +14,11 @@ namespace dev {
 
 // WPE: waves per SIMD the register allocation must leave room for (two
 // workgroups of C * RB threads per CU: C * RB / 128).
-template <typename T, int C, int RA, int RB, int NCH, int WPE, int TWM>
+// PAIR: the instance with the DC / Nyquist pair modes of a 2D real column
+// pass (a separate instance keeps that code out of every other pass).
+template <typename T, int C, int RA, int RB, int NCH, int WPE, int TWM, bool PAIR>
 __global__ __launch_bounds__(C * RB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_col(const StageArgs a) {
-  col_body<T, C, RA, RB, NCH, TWM>(a, remap_block(blockIdx.x, a.nblocks, a.xcd_remap));
+  col_body<T, C, RA, RB, NCH, TWM, PAIR>(a, remap_block(blockIdx.x, a.nblocks, a.xcd_remap));
 }
 
 }  // namespace dev
@@ -29,12 +31,18 @@ struct ColEntry {
   const void *f;
   int pref;  // default instance for first passes (1), later passes (2), both (3), neither (0)
   int qvar;  // the persistent queue kernel's variant of this instance (k_queue in kernels.hip), -1 if none
+  const void *fp;  // the instance with the DC / Nyquist pair modes, nullptr if none
 };
 
-#define FLAGFFT_COL(T, C, RA, RB, NCH, WPE, PREF) \
-  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1>, PREF, -1}
-#define FLAGFFT_COLQ(T, C, RA, RB, NCH, WPE, PREF, QVAR) \
-  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1>, PREF, QVAR}
+#define FLAGFFT_COL(T, C, RA, RB, NCH, WPE, PREF)                                                                  \
+  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1, false>, PREF, \
+   -1, nullptr}
+#define FLAGFFT_COLP(T, C, RA, RB, NCH, WPE, PREF)                                                                 \
+  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1, false>, PREF, \
+   -1, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1, true>}
+#define FLAGFFT_COLQ(T, C, RA, RB, NCH, WPE, PREF, QVAR)                                                           \
+  {RA * RB, RA, RB, C, NCH, WPE, sizeof(T) == 8, (const void *)dev::k_col<T, C, RA, RB, NCH, WPE, 1, false>, PREF, \
+   QVAR, nullptr}
 
 // Measured on MI355X (2^20 x 16 and 2^24 C2C FP32): for R = 4096 (64 x 64
 // in registers, 198 VGPRs, one 512-thread workgroup per CU) 8 units per
@@ -46,7 +54,7 @@ struct ColEntry {
 // units (65-67 us per 2^24 pass against 72-73 with 64), later passes with 64
 // units in two exchange chunks (55-59 us against 62-65 with 16).
 const ColEntry kColTable[] = {
-    FLAGFFT_COL(float, 8, 64, 64, 2, 2, 3),           // 4096: 64-byte runs, 135 KB LDS, 198 VGPRs
+    FLAGFFT_COLP(float, 8, 64, 64, 2, 2, 3),          // 4096: 64-byte runs, 135 KB LDS, 198 VGPRs
     FLAGFFT_COL(float, 16, 32, 32, 2, 4, 2),          // 1024: 128-byte runs, 68 KB LDS, two workgroups per CU
     FLAGFFT_COL(float, 32, 32, 32, 2, 4, 1),          // 1024: 256-byte runs, 136 KB LDS, 1024 threads
     FLAGFFT_COL(float, 16, 32, 32, 1, 2, 0),          // 1024: 132 KB LDS, one workgroup per CU
@@ -131,10 +139,17 @@ void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &
 
 const void *col_kernel(int id) { return kColTable[id].f; }
 
-std::string col_kernel_targs(int id) {
+const void *col_pair_kernel(int id) { return kColTable[id].fp; }
+
+int col_pair_kernel_id(int64_t r, bool dbl) {
+  const int id = col_kernel_id(r, dbl, 0, true);
+  return id >= 0 && kColTable[id].fp != nullptr ? id : -1;
+}
+
+std::string col_kernel_targs(int id, bool pair) {
   const ColEntry &e = kColTable[id];
   return std::to_string(e.c) + "," + std::to_string(e.ra) + "," + std::to_string(e.rb) + "," + std::to_string(e.nch) +
-         "," + std::to_string(e.wpe) + ",1";
+         "," + std::to_string(e.wpe) + ",1," + (pair ? "true" : "false");
 }
 
 }  // namespace flagfft

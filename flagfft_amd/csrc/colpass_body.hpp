@@ -65,7 +65,7 @@ __device__ __forceinline__ void pow8(V2<T> w, V2<T> *p) {
 
 // Loads of tile `blk`: element e = t + RB j of the thread's unit (zero
 // outside the units and, for a Bluestein chirp source, past the user length).
-template <typename T, int C, int RA, int RB>
+template <typename T, int C, int RA, int RB, bool PAIR = false>
 __device__ __forceinline__ void col_load(const StageArgs &a, int blk, V2<T> (&x)[RA]) {
   const int tid = threadIdx.x;
   const int sl = tid % C;
@@ -78,6 +78,24 @@ __device__ __forceinline__ void col_load(const StageArgs &a, int blk, V2<T> (&x)
   const GlbC<T> *src = gc<T>(a.in) + static_cast<int64_t>(bq) * a.bs_in;
   const bool chirp_in = (a.src_mode & kSrcChirpFlag) != 0;
   const uint32_t nlog = static_cast<uint32_t>(a.nlog);
+  if (PAIR && a.dc_nyq_pair == 2 && blk == 0 && sl == 0) {
+    // unit 0 of an inverse 2D real column pass: Herm(A) + i Herm(B) for the
+    // DC column A (column 0) and the Nyquist column B (column `units`)
+    constexpr int R = RA * RB;
+    const uint32_t cb = static_cast<uint32_t>(a.units);
+    const T half = T(0.5);
+#pragma unroll
+    for (int j = 0; j < RA; ++j) {
+      const uint32_t e = static_cast<uint32_t>(t + RB * j);
+      const uint32_t em = e == 0 ? 0u : static_cast<uint32_t>(R) - e;
+      const V2<T> va = ldt(src, e * tcnt), vam = ldt(src, em * tcnt);
+      const V2<T> vb = ldt(src, cb + e * tcnt), vbm = ldt(src, cb + em * tcnt);
+      const V2<T> ha{(va.x + vam.x) * half, (va.y - vam.y) * half};
+      const V2<T> hb{(vb.x + vbm.x) * half, (vb.y - vbm.y) * half};
+      x[j] = V2<T>{ha.x - hb.y, ha.y + hb.x};
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < RA; ++j) {
     const uint32_t e = i + static_cast<uint32_t>(t + RB * j) * tcnt;
@@ -88,7 +106,8 @@ __device__ __forceinline__ void col_load(const StageArgs &a, int blk, V2<T> (&x)
 // Everything after the loads of tile `blk` (x: what col_load gave).
 // TWM: twiddles from table loads per power (0) or from two or three loads and
 // products (1: w^j = w^(j mod 8) (w^8)^(j / 8), fewer memory instructions).
-template <typename T, int C, int RA, int RB, int NCH, int TWM>
+// PAIR: the instance also runs the DC / Nyquist pair modes (StageArgs::dc_nyq_pair).
+template <typename T, int C, int RA, int RB, int NCH, int TWM, bool PAIR = false>
 __device__ __forceinline__ void col_rest(const StageArgs &a, int blk, V2<T> (&x)[RA]) {
   constexpr int R = RA * RB;
   constexpr int NM = RA / RB;   // stage-B transforms per thread
@@ -332,23 +351,59 @@ __device__ __forceinline__ void col_rest(const StageArgs &a, int blk, V2<T> (&x)
         }
       }
     }
-  } else if (active) {
-    GlbC<T> *dst = gc<T>(a.out) + static_cast<int64_t>(bq) * a.bs_out;
-    const uint32_t o0 = (i - k) * static_cast<uint32_t>(R) + k;
-    const uint32_t pp = static_cast<uint32_t>(a.p);
-    const GlbC<T> *bhat = gc<T>(a.bhat);
+  } else {
+    // forward 2D real column pass with the DC / Nyquist pair in unit 0: its
+    // outputs go through LDS so that each thread sees Z_e and Z_-e
+    // (workgroup 0 only, uniform over the workgroup)
+    const bool unpack = PAIR && a.dc_nyq_pair == 1 && blk == 0;
+    if (unpack) {
+      __syncthreads();  // the exchange's reads are done
+      if (sl == 0) {
 #pragma unroll
-    for (int m = 0; m < NM; ++m) {
-      const int k1 = t + RB * m;
+        for (int m = 0; m < NM; ++m) {
 #pragma unroll
-      for (int q = 0; q < RB; ++q) {
-        const uint32_t o = o0 + static_cast<uint32_t>(k1 + RA * q) * pp;
-        if (a.dst_mode & kDstPointwiseFlag) {
-          stt(dst, o, cconj(cmul(y[m][q], ldt(bhat, o))));
-        } else if (a.dst_mode & kDstChirpFlag) {
-          if (o < nlog) stt(dst, o, finish(cmul(cconj(y[m][q]), ldt(chirp, o))));
-        } else {
-          stt(dst, o, finish(y[m][q]));
+          for (int q = 0; q < RB; ++q) stc(lds + (t + RB * m + RA * q), y[m][q]);
+        }
+      }
+      __syncthreads();
+    }
+    if (active) {
+      GlbC<T> *dst = gc<T>(a.out) + static_cast<int64_t>(bq) * a.bs_out;
+      const uint32_t o0 = (i - k) * static_cast<uint32_t>(R) + k;
+      const uint32_t pp = static_cast<uint32_t>(a.p);
+      const GlbC<T> *bhat = gc<T>(a.bhat);
+      if (unpack && sl == 0) {
+        // unit 0 is column 0 of the single pass (o0 = 0); B goes to column `units`
+        const uint32_t cb = static_cast<uint32_t>(a.units);
+        const T half = T(0.5);
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+#pragma unroll
+          for (int q = 0; q < RB; ++q) {
+            const int e = t + RB * m + RA * q;
+            const V2<T> z = y[m][q];
+            const V2<T> zm = ldc(lds + (e == 0 ? 0 : R - e));  // Z_-e (conjugated below)
+            const V2<T> va{(z.x + zm.x) * half, (z.y - zm.y) * half};
+            const V2<T> vb{(z.y + zm.y) * half, (zm.x - z.x) * half};
+            stt(dst, static_cast<uint32_t>(e) * pp, finish(va));
+            stt(dst, cb + static_cast<uint32_t>(e) * pp, finish(vb));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+          const int k1 = t + RB * m;
+#pragma unroll
+          for (int q = 0; q < RB; ++q) {
+            const uint32_t o = o0 + static_cast<uint32_t>(k1 + RA * q) * pp;
+            if (a.dst_mode & kDstPointwiseFlag) {
+              stt(dst, o, cconj(cmul(y[m][q], ldt(bhat, o))));
+            } else if (a.dst_mode & kDstChirpFlag) {
+              if (o < nlog) stt(dst, o, finish(cmul(cconj(y[m][q]), ldt(chirp, o))));
+            } else {
+              stt(dst, o, finish(y[m][q]));
+            }
+          }
         }
       }
     }
@@ -357,11 +412,11 @@ __device__ __forceinline__ void col_rest(const StageArgs &a, int blk, V2<T> (&x)
 
 // One tile (workgroup `blk` of the launch) of a register-resident pass; every
 // one of the C * RB threads of the workgroup must call it (it synchronises).
-template <typename T, int C, int RA, int RB, int NCH, int TWM>
+template <typename T, int C, int RA, int RB, int NCH, int TWM, bool PAIR = false>
 __device__ __forceinline__ void col_body(const StageArgs &a, int blk) {
   V2<T> x[RA];
-  col_load<T, C, RA, RB>(a, blk, x);
-  col_rest<T, C, RA, RB, NCH, TWM>(a, blk, x);
+  col_load<T, C, RA, RB, PAIR>(a, blk, x);
+  col_rest<T, C, RA, RB, NCH, TWM, PAIR>(a, blk, x);
 }
 
 }  // namespace dev

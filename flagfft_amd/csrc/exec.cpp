@@ -209,7 +209,7 @@ void Executable::resolve_kernels() {
       case Step::Pass: s.func = pass_kernel(s.dbl, s.ept); break;
       case Step::Leaf: s.func = leaf_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
       case Step::Stage: s.func = stage_kernel(s.dbl, s.leaf_id, s.leaf_mode); break;
-      case Step::Col: s.func = col_kernel(s.leaf_id); break;
+      case Step::Col: s.func = s.leaf_mode != 0 ? col_pair_kernel(s.leaf_id) : col_kernel(s.leaf_id); break;
       case Step::Mfma: s.func = mfma_kernel(s.compute, s.leaf_id); break;
       case Step::BluesteinLds: s.func = bluestein_lds_kernel(s.leaf_id, s.leaf_mode); break;
       case Step::BluesteinCombine: s.func = bluestein_combine_kernel(s.dbl, s.split); break;
@@ -369,6 +369,13 @@ Builder::Builder(Prec p, const PlanOptions &o, Executable &ex, hipStream_t setup
 
 Builder::~Builder() {
   if (!temps_.empty() || !temp_execs_.empty()) (void)hipStreamSynchronize(st_);
+}
+
+int Builder::set_dc_nyq_pair(int mode) {
+  const int n = pair_steps_;
+  pair_ = mode;
+  pair_steps_ = 0;
+  return n;
 }
 
 void Builder::finish() {
@@ -751,29 +758,47 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
   int cid = !col_allowed        ? -1
             : opt_.queue_form ? col_queue_kernel_id(a.R, dbl())
                               : col_kernel_id(a.R, dbl(), opt_.forced_units, a.p == 1);
+  // DC / Nyquist pair: one transform's single column pass over the n1 / 2
+  // columns of the half spectrum (row pitch n1 / 2 + 1) but its Nyquist
+  // column, which goes with column 0; the other side is a workspace of n1 / 2
+  // columns per row (row pitch at least that)
+  const bool pair = pair_ != 0;
+  const int64_t pair_units = pair ? (pair_ == 1 ? a.go.es : a.gi.es) - 1 : 0;
+  if (pair) {
+    if (!col_allowed || !s.col_fast || s.nbatch != 1 || npasses != 1 || a.tcnt != 1 || a.gi.cs != 1 ||
+        a.go.cs != 1 || (pair_ == 1 ? a.gi.es : a.go.es) < pair_units || pair_units < 1)
+      throw std::logic_error("DC/Nyquist pair: not a single column pass of one transform");
+    cid = col_pair_kernel_id(a.R, dbl());
+    if (cid < 0) throw std::logic_error("DC/Nyquist pair: no paired column instance");
+  }
   if ((sid < 0 && cid < 0) || (a.src_mode & 3) != kSrcComplex || (a.dst_mode & 3) != kDstComplex) return false;
   // contiguous sequences, or the columns of contiguous rows of ncol elements
   const bool rows = !s.col_fast && s.ncol == 1 && a.gi.es == 1 && a.go.es == 1;
-  const bool cols = s.col_fast && s.ncol > 1 && a.gi.es == s.ncol && a.gi.cs == 1 && a.go.es == s.ncol &&
-                    a.go.cs == 1 && mode == 0 && a.src_mode == kSrcComplex && a.dst_mode == kDstComplex;
+  const bool cols = pair || (s.col_fast && s.ncol > 1 && a.gi.es == s.ncol && a.gi.cs == 1 && a.go.es == s.ncol &&
+                             a.go.cs == 1 && mode == 0 && a.src_mode == kSrcComplex && a.dst_mode == kDstComplex);
   if (!rows && !cols) return false;
-  const int64_t ncol = cols ? s.ncol : 1;
+  // pair: units read at input pitch tcnt and written at output pitch p (one
+  // transform, so every unit index is below both)
+  const int64_t ncol = pair ? a.go.es : cols ? s.ncol : 1;
   const int R = a.R;
-  const int64_t tcnt = a.tcnt * ncol, p = a.p * ncol;
-  if (tcnt >= (int64_t{1} << 31)) return false;
+  const int64_t tcnt = pair ? a.gi.es : a.tcnt * ncol, p = pair ? a.go.es : a.p * ncol;
+  const int64_t units = pair ? pair_units : a.units;
+  if (std::max(tcnt, p) >= (int64_t{1} << 31)) return false;
   int cunits = 0, cthreads = 0, cebytes = 0;
   size_t clds = 0;
   if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
   // the default instance's tile does not divide the units of a sequence:
   // another instance of the radix whose tile does
-  if (cid >= 0 && tcnt % cunits != 0 && a.units != tcnt && opt_.forced_units == 0 && !opt_.queue_form) {
+  if (cid >= 0 && !pair && tcnt % cunits != 0 && a.units != tcnt && opt_.forced_units == 0 && !opt_.queue_form) {
     cid = col_kernel_id_dividing(R, dbl(), tcnt);
     if (cid >= 0) col_kernel_shape(cid, cunits, cthreads, clds, cebytes);
   }
   // the register-resident pass addresses a sequence from a uniform base with
   // 32-bit offsets: its tiles may not straddle sequences, nor a sequence 4 GiB
   // (one sequence: the last tile may be partial, its lanes past the units idle)
-  const bool col_ok = cid >= 0 && (tcnt % cunits == 0 || a.units == tcnt) && tcnt * R * eb() < (int64_t{1} << 32);
+  const bool col_ok = cid >= 0 && (pair || tcnt % cunits == 0 || a.units == tcnt) &&
+                      std::max(tcnt, p) * R * eb() < (int64_t{1} << 32);
+  if (pair && !col_ok) throw std::logic_error("DC/Nyquist pair: the column pass does not fit");
   if (!col_ok && sid < 0) return false;
   if (col_ok) {
     Step cst;
@@ -799,25 +824,32 @@ bool Builder::try_stage(const PassArgs &a, const std::vector<int> &radices, int 
     g.conj_in = a.conj_in;
     g.conj_out = a.conj_out;
     g.scale = a.scale;
-    g.units = a.units;
+    g.units = units;
     g.block_store = (rows && p == 1 && a.go.bs == n && a.dst_mode == kDstComplex) ? 1 : 0;
     g.src_mode = a.src_mode;
     g.dst_mode = a.dst_mode;
     g.nlog = a.nlog;
     g.chirp = a.chirp;
     g.bhat = a.bhat;
-    const int64_t nblocks = (a.units + cunits - 1) / cunits;
+    g.dc_nyq_pair = pair_;
+    if (pair) {
+      cst.leaf_mode = 1;  // the paired instance (Executable::resolve_kernels)
+      ++pair_steps_;
+    }
+    const int64_t nblocks = (g.units + cunits - 1) / cunits;
     g.nblocks = static_cast<int32_t>(nblocks);
     g.xcd_remap = opt_.xcd_remap;
     cst.ept = 0;
     cst.shape = LaunchShape{dim3(static_cast<unsigned>(nblocks)), dim3(static_cast<unsigned>(cthreads)), clds};
-    cst.bytes_moved =
-        s.nbatch * ncol * (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
+    // (pair: the n1 / 2 + 1 half-spectrum columns' bytes on either side)
+    cst.bytes_moved = s.nbatch * (pair ? units + 1 : ncol) *
+                      (src_bytes(a.src_mode, n, a.nlog, eb()) + dst_bytes(a.dst_mode, n, a.nlog, eb()));
     std::ostringstream lab;
-    lab << "k_col<" << (dbl() ? "fp64" : "fp32") << "," << col_kernel_targs(cid) << "> pass " << pass << "/" << npasses
-        << " R=" << R << " p=" << p << " tcnt=" << tcnt << (g.block_store ? " block-store" : "")
+    lab << "k_col<" << (dbl() ? "fp64" : "fp32") << "," << col_kernel_targs(cid, pair) << "> pass " << pass << "/"
+        << npasses << " R=" << R << " p=" << p << " tcnt=" << tcnt << (g.block_store ? " block-store" : "")
         << (cols ? " columns=" : "");
     if (cols) lab << ncol;
+    if (pair) lab << (pair_ == 1 ? " dc-nyquist-unpack" : " dc-nyquist-pack");
     cst.label = lab.str();
     ex_.steps.push_back(cst);
     return true;
@@ -1473,7 +1505,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     return;
   }
   const int64_t h = n / 2;
-  if (pl.real_form == RealForm::EvenFused && try_leaf_split_real(pl, s, src, dst)) return;
+  if (pl.real_form == RealForm::EvenFused && pair_ == 0 && try_leaf_split_real(pl, s, src, dst)) return;
   if (pl.real_form == RealForm::EvenFused) {
     const int lid = opt_.generic_kernels ? -1 : leaf_kernel_id(pl.core.radices);
     if (lid >= 0 && s.ncol == 1 && !s.col_fast && s.gi.es == 1 && s.go.es == 1 && h > 1 &&
@@ -1499,6 +1531,8 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
       a.conj_in = 0;
       a.conj_out = pl.real_fwd ? 0 : 1;
       a.scale = 1.0;
+      a.dc_nyq_pair = pair_ != 0 ? 1 : 0;
+      if (pair_ != 0) ++pair_steps_;
       int C = std::max(1, opt_.target_threads / tpu);
       if ((s.nbatch + C - 1) / C < 512) C = static_cast<int>(std::max<int64_t>(1, s.nbatch / 512));
       if (opt_.forced_units > 0) C = opt_.forced_units;
@@ -1514,7 +1548,8 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
       std::ostringstream lab;
       lab << "k_leaf<" << (dbl() ? "fp64" : "fp32") << "," << ept << "," << (pl.real_fwd ? 1 : 2) << ">[";
       for (size_t i = 0; i < pl.core.radices.size(); ++i) lab << (i ? "x" : "") << pl.core.radices[i];
-      lab << "] " << (pl.real_fwd ? "R2C" : "C2R") << " n=" << n << " half=" << h << " C=" << C << " tpu=" << tpu;
+      lab << "] " << (pl.real_fwd ? "R2C" : "C2R") << " n=" << n << " half=" << h << " C=" << C << " tpu=" << tpu
+          << (pair_ != 0 ? " dc-nyquist-pair" : "");
       st.label = lab.str();
       ex_.steps.push_back(st);
       return;
@@ -1735,34 +1770,44 @@ std::unique_ptr<Executable> build_executable(const PlanTree &t, bool inverse, co
     } else if (t.desc.real_fwd) {
       // rows into a workspace (an in-place call would otherwise rewrite the
       // buffer at the complex row pitch while other rows are still read),
-      // columns from it into the output: the same two sweeps
-      const int W1 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
+      // columns from it into the output: the same two sweeps.  With the DC /
+      // Nyquist pair the workspace rows hold n1 / 2 bins (Re X_N in Im X_0)
+      // at a pitch of n1 / 2 + 128 bytes: whole 128-byte lines that the
+      // column pass reads as aligned segments, rows not a power of two apart
+      const int64_t wp = t.dc_nyq_pair ? n1 / 2 + 128 / elem_bytes(t.desc.prec) : h1;
+      const int W1 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * wp) * elem_bytes(t.desc.prec));
       SeqSet r;
       r.nbatch = batch * n0;
       r.gi = Geometry{1, 0, n1};
-      r.go = Geometry{1, 0, h1};
+      r.go = Geometry{1, 0, wp};
+      b.set_dc_nyq_pair(t.dc_nyq_pair ? 1 : 0);
       b.emit_plan1d(t.rows, r, kBufIn, W1, false);
       SeqSet c;
       c.nbatch = batch;
       c.ncol = h1;
-      c.gi = Geometry{h1, 1, n0 * h1};
-      c.go = c.gi;
+      c.gi = Geometry{wp, 1, n0 * wp};
+      c.go = Geometry{h1, 1, n0 * h1};
       c.col_fast = true;
       b.emit_plan1d(t.cols, c, W1, kBufOut, false);
+      if (b.set_dc_nyq_pair(0) != (t.dc_nyq_pair ? 2 : 0)) throw std::logic_error("DC/Nyquist pair: unpaired launch");
     } else {
-      const int W2 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * h1) * elem_bytes(t.desc.prec));
+      // (the paired column pass writes n1 / 2 columns per workspace row)
+      const int64_t wp = t.dc_nyq_pair ? n1 / 2 + 128 / elem_bytes(t.desc.prec) : h1;
+      const int W2 = b.pick_ws({kBufIn, kBufOut}, static_cast<size_t>(batch * n0 * wp) * elem_bytes(t.desc.prec));
       SeqSet c;
       c.nbatch = batch;
       c.ncol = h1;
       c.gi = Geometry{h1, 1, n0 * h1};
-      c.go = c.gi;
+      c.go = Geometry{wp, 1, n0 * wp};
       c.col_fast = true;
+      b.set_dc_nyq_pair(t.dc_nyq_pair ? 2 : 0);
       b.emit_plan1d(t.cols, c, kBufIn, W2, true);
       SeqSet r;
       r.nbatch = batch * n0;
-      r.gi = Geometry{1, 0, h1};
+      r.gi = Geometry{1, 0, wp};
       r.go = Geometry{1, 0, n1};
       b.emit_plan1d(t.rows, r, W2, kBufOut, true);
+      if (b.set_dc_nyq_pair(0) != (t.dc_nyq_pair ? 2 : 0)) throw std::logic_error("DC/Nyquist pair: unpaired launch");
     }
   }
   b.finish();

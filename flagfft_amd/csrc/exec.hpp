@@ -162,6 +162,10 @@ class Builder {
   void emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, bool inverse);
   int pick_ws(std::initializer_list<int> avoid, size_t bytes);
   void finish();  // wait for table generation, drop temporaries
+  // DC / Nyquist pair mode of the real-row leaf and the column pass emitted
+  // next (PlanTree::dc_nyq_pair; 1 forward, 2 inverse, 0 off); returns the
+  // number of launches emitted in that mode since the last call.
+  int set_dc_nyq_pair(int mode);
 
  private:
   Prec prec_;
@@ -170,6 +174,7 @@ class Builder {
   hipStream_t st_;
   std::vector<std::shared_ptr<DevBuf>> temps_;
   std::vector<std::unique_ptr<Executable>> temp_execs_;
+  int pair_ = 0, pair_steps_ = 0;
 
   bool dbl() const { return prec_ == Prec::F64; }
   int eb() const { return elem_bytes(prec_); }

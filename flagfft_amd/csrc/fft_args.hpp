@@ -128,6 +128,10 @@ struct LeafArgs {
   int32_t conj_in, conj_out;
   double scale;
   int32_t C;             // transforms per workgroup
+  // real rows of a 2D transform with one column set: bin N's real part rides
+  // in the imaginary part of bin 0 (R2C writes it there and leaves bin N
+  // unwritten, C2R reads it there; both bins are real on that side)
+  int32_t dc_nyq_pair;
 };
 
 // Single-pass transform of length N = P L spread over P workgroups per
@@ -170,6 +174,13 @@ struct StageArgs {
   int32_t src_mode, dst_mode;  // complex, or the Bluestein chirp source / pointwise and chirp sinks
   int64_t nlog;           // user length for the chirp modes
   const void *chirp, *bhat;
+  // single column pass of a 2D real transform whose DC column A and Nyquist
+  // column B (column `units`) share unit 0 as A + i B:
+  //   1: forward, A and B real: split unit 0's output Z into
+  //      FFT A = (Z_e + conj Z_-e) / 2 (column 0) and FFT B = -i (Z_e - conj Z_-e) / 2
+  //   2: inverse: unit 0 reads Herm(A) + i Herm(B), Herm(V)_e = (V_e + conj V_-e) / 2,
+  //      whose inverse transform is Re IFFT A + i Re IFFT B
+  int32_t dc_nyq_pair;
 };
 
 // Bluestein with the whole convolution in LDS (one sequence per workgroup).

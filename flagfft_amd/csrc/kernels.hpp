@@ -58,12 +58,17 @@ int col_run_bytes(int64_t r, bool dbl, bool first);
 int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt);
 void col_kernel_shape(int id, int &units, int &threads, size_t &lds_bytes, int &elem_bytes);
 const void *col_kernel(int id);
+// The same instance with the DC / Nyquist pair modes of a 2D real column pass
+// (StageArgs::dc_nyq_pair), null if it has none; the first-pass default
+// instance of r when it has one, else -1.
+const void *col_pair_kernel(int id);
+int col_pair_kernel_id(int64_t r, bool dbl);
 // The instance of 256 threads the persistent queue kernel also runs (-1 if
 // none for r), and its queue variant (QueueJob::variant; -1 for others).
 int col_queue_kernel_id(int64_t r, bool dbl);
 int col_queue_variant(int id);
 // Template arguments after the element type, as the kernel symbol lists them.
-std::string col_kernel_targs(int id);
+std::string col_kernel_targs(int id, bool pair = false);
 
 // Persistent mixed-length kernel (kQueueThreads threads per workgroup, one
 // QueueArgs argument).

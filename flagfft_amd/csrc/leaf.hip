@@ -27,6 +27,7 @@ namespace dev {
 //   kLeafC2R      Hermitian rows of N+1 bins: Z'_k = (X_k + conj X_{N-k})
 //                 + i conj(w_n^k) (X_k - conj X_{N-k}) (Im of X_0, X_N dropped),
 //                 z = conj FFT_N(conj Z') -> (x_2j, x_2j+1)
+//   (LeafArgs::dc_nyq_pair: Re X_N travels as Im X_0, bin N is not accessed)
 // (python/flagfft_codegen/kernels.py:1098-1221 promotes real rows to full-length
 // complex transforms instead; the split-spectrum identity halves the work.)
 
@@ -68,7 +69,8 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
       const int k = t + i * TPU;
       const bool ok = active && k <= N / 2;
       xks[i] = ok ? ldg(src + k) : V2<T>{0, 0};
-      xhs[i] = ok ? ldg(src + (N - k)) : V2<T>{0, 0};
+      // a paired row carries X_N's real part in the imaginary part of X_0
+      xhs[i] = (ok && !(k == 0 && a.dc_nyq_pair)) ? ldg(src + (N - k)) : V2<T>{0, 0};
     }
 #pragma unroll
     for (int i = 0; i < NK; ++i) {
@@ -77,6 +79,7 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
       V2<T> xk = xks[i];
       V2<T> xh = xhs[i];
       if (k == 0) {
+        if (a.dc_nyq_pair) xh.x = xk.y;
         xk.y = T(0);
         xh.y = T(0);
       }
@@ -168,8 +171,12 @@ __global__ __launch_bounds__(kMaxBlockThreads) void k_leaf(const LeafArgs a) {
           lo.y = -lo.y;
           hi.y = -hi.y;
         }
-        stg(dst + k, lo);
-        if (2 * k != N) stg(dst + (N - k), hi);
+        if (k == 0 && a.dc_nyq_pair) {
+          stg(dst, V2<T>{lo.x, hi.x});  // X_0 and X_N are real: one slot
+        } else {
+          stg(dst + k, lo);
+          if (2 * k != N) stg(dst + (N - k), hi);
+        }
       }
     } else if (active) {
       // last stage: q * RR == N, so k == b and the output index is b + j q

@@ -85,6 +85,10 @@ struct PlanTree {
   Plan1D rows;   // rank 1: the transform; rank 2 / 3: along the last axis (contiguous)
   Plan1D mid;    // rank 3: C2C along n1 (strided)
   Plan1D cols;   // rank 2 / 3: C2C along n0 (strided)
+  // rank 2 real, one transform: the DC and Nyquist half-spectrum columns
+  // share one unit of a single register-resident column pass (n1 / 2 units
+  // of work instead of n1 / 2 + 1, which would leave one tile alone)
+  bool dc_nyq_pair = false;
   std::string describe() const;
 };
 
@@ -106,6 +110,7 @@ int leaf_kernel_id(const std::vector<int> &radices);
 int col_kernel_id(int64_t r, bool dbl, int units);
 int col_queue_kernel_id(int64_t r, bool dbl);
 int col_run_bytes(int64_t r, bool dbl, bool first);
+int col_pair_kernel_id(int64_t r, bool dbl);
 // True when every super-radix of the split has a specialised kernel.
 bool passes_specialised(const std::vector<int64_t> &passes);
 

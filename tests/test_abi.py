@@ -254,10 +254,17 @@ def test_planner_choices(untuned):
     assert "super_radices=[1024,1024]" in _describe(1 << 20)
     assert "TwoDim(n0=4096, n1=4096" in _describe([4096, 4096], T.R2C, rank=2)
     # 4096-point columns: one register-resident pass when its 8-column tiles
-    # fill the CUs in whole rounds (4096 columns), two 64-point passes for
-    # the 2049 half-spectrum columns
+    # fill the CUs in whole rounds (4096 columns); the 2049 half-spectrum
+    # columns of one real transform take it with the DC and Nyquist columns
+    # sharing a unit (2048 units)
     assert "super_radices=[64,64]" not in _describe([4096, 4096], T.C2C, rank=2)
-    assert "super_radices=[64,64]" in _describe([4096, 4096], T.R2C, rank=2)
+    for t in (T.R2C, T.C2R):
+        d = _describe([4096, 4096], t, rank=2)
+        assert "dc_nyquist_pair" in d and "super_radices=[64,64]" not in d, d
+    assert "dc_nyquist_pair" in _describe([4096, 8192], T.R2C, rank=2)
+    assert "dc_nyquist_pair" not in _describe([4096, 4094], T.R2C, rank=2)  # 2048 columns: whole rounds
+    assert "dc_nyquist_pair" not in _describe([4096, 4096], T.C2C, rank=2)
+    assert "dc_nyquist_pair" not in _describe([2048, 4096], T.R2C, rank=2)  # no 2048-point column instance
 
 
 def test_tuning_overrides_reach_the_planner(untuned):

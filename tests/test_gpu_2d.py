@@ -206,18 +206,28 @@ def test_4096_point_columns(gpu, shape, batch, single):
                    what=f"2D {n0}x{n1} batch {batch}")
 
 
-def test_4096_point_columns_of_real_halves(gpu):
-    """R2C / C2R 4096 x 4096: 2049 half-spectrum columns would leave one tile
-    of the single-pass form alone on the GPU; they take two 64-point passes."""
-    n0 = n1 = 4096
-    x = real_input(0x4096, (1, n0, n1), np.float32, kind="normal")
-    xd = dev_tensor(x, gpu)
-    h = n1 // 2 + 1
+# One real transform's n1 / 2 + 1 half-spectrum columns of 4096 points would
+# leave one 8-column tile alone on the GPU: the DC and Nyquist columns (real on
+# the real side) share unit 0 of a single register-resident pass instead
+# (R2C: split after the transform; C2R: the Hermitian parts of both columns
+# enter it); several transforms take two 64-point passes.
+@pytest.mark.parametrize("shape,batch,paired", [((4096, 4096), 1, True), ((4096, 8192), 1, True),
+                                                ((4096, 4096), 2, False)],
+                         ids=["4096x4096", "4096x8192", "4096x4096-b2"])
+def test_4096_point_columns_of_real_halves(gpu, shape, batch, paired):
     import torch
-    yd = torch.zeros((1, n0, h), dtype=torch.complex64, device=gpu)
+
+    n0, n1 = shape
+    h = n1 // 2 + 1
+    x = real_input(0x4096 + n1 + batch, (batch, n0, n1), np.float32, kind="normal")
+    xd = dev_tensor(x, gpu)
+    yd = torch.zeros((batch, n0, h), dtype=torch.complex64, device=gpu)
     zd = torch.zeros_like(xd)
-    with ff.Plan.two_d(n0, n1, ff.Type.R2C) as pf, ff.Plan.two_d(n0, n1, ff.Type.C2R) as pi:
-        assert "super_radices=[64,64]" in pf.description()
+    with _plan(n0, n1, ff.Type.R2C, batch) as pf, _plan(n0, n1, ff.Type.C2R, batch) as pi:
+        df, di = pf.description(), pi.description()
+        assert ("dc_nyquist_pair" in df and "dc-nyquist-unpack" in df) == paired, df
+        assert ("dc_nyquist_pair" in di and "dc-nyquist-pack" in di) == paired, di
+        assert ("super_radices=[64,64]" in df) != paired, df
         for p in (pf, pi):
             p.set_stream(stream_of(gpu))
         pf.r2c(xd.data_ptr(), yd.data_ptr())
@@ -225,7 +235,30 @@ def test_4096_point_columns_of_real_halves(gpu):
         sync(gpu)
     n = n0 * n1
     ref = ora.rfft2(x.astype(np.float64))
-    check_accuracy(host(yd).reshape(1, -1), ref.reshape(1, -1), ff.Type.R2C, n, row_len=n0 * h, factor=2.0,
-                   what="2D R2C 4096x4096")
-    check_accuracy(host(zd).reshape(1, -1), n * x.astype(np.float64).reshape(1, -1), ff.Type.C2R, n, row_len=n,
-                   factor=4.0, what="2D C2R roundtrip 4096x4096")
+    X = host(yd)
+    check_accuracy(X.reshape(batch, -1), ref.reshape(batch, -1), ff.Type.R2C, n, row_len=n0 * h, factor=2.0,
+                   what=f"2D R2C {n0}x{n1} batch {batch}")
+    for c in (0, 1, h - 2, h - 1):  # the paired columns and their neighbours, each on its own
+        check_accuracy(X[:, :, c], ref[:, :, c], ff.Type.R2C, n, row_len=n0, factor=2.0, what=f"column {c}")
+    check_accuracy(host(zd).reshape(batch, -1), n * x.astype(np.float64).reshape(batch, -1), ff.Type.C2R, n,
+                   row_len=n, factor=4.0, what=f"2D C2R roundtrip {n0}x{n1} batch {batch}")
+
+
+def test_c2r_2d_paired_columns_of_non_hermitian_input(gpu):
+    """C2R 4096 x 4096 of a non-Hermitian spectrum through the paired DC /
+    Nyquist unit: the result is the reference's (column transforms of every
+    half-spectrum column, then each row's C2R drops Im of its DC and Nyquist
+    bins), which the paired unit reaches through the Hermitian parts of the two
+    columns."""
+    n0 = n1 = 4096
+    h = n1 // 2 + 1
+    X = complex_input(0xC2F, (1, n0, h), np.complex64, kind="normal")
+    Xd = dev_tensor(X, gpu)
+    yd = dev_tensor(np.zeros((1, n0, n1), np.float32), gpu)
+    with ff.Plan.two_d(n0, n1, ff.Type.C2R) as p:
+        assert "dc-nyquist-pack" in p.description()
+        p.set_stream(stream_of(gpu))
+        p.c2r(Xd.data_ptr(), yd.data_ptr())
+        sync(gpu)
+    check_accuracy(host(yd), ora.irfft2(X.astype(np.complex128), n1), ff.Type.C2R, n0 * n1, row_len=n0 * n1,
+                   factor=2.0, what="non-Hermitian C2R 4096x4096")
