@@ -560,7 +560,10 @@ ShapeChoice pick_shape(const PlanOptions &o, int eb, int64_t work_len, int LS, b
     int tpu = static_cast<int>((work_len + ept - 1) / ept);
     if (tpu < 1) tpu = 1;
     if (tpu > o.max_threads) continue;
-    int desired = chunked ? std::max(1, o.chunk_bytes / eb) : std::max(1, o.target_threads / tpu);
+    // strided passes: at least chunk_bytes of adjacent units per row segment,
+    // and (short units of few threads) at least target_threads threads
+    int desired = chunked ? std::max({1, o.chunk_bytes / eb, o.target_threads / tpu})
+                          : std::max(1, o.target_threads / tpu);
     if (!chunked && (nunits + desired - 1) / desired < kTargetGroups)
       desired = static_cast<int>(std::max<int64_t>(1, nunits / kTargetGroups));
     if (o.forced_units > 0) desired = o.forced_units;

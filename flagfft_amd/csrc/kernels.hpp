@@ -45,6 +45,9 @@ int stage_kernel_id(const std::vector<int> &radices);
 int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units);
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept);
 const void *stage_kernel(bool dbl, int id, int mode);  // mode: 0 chain pass, 1 Bluestein convolution
+// False for the instances that beat only the run-time-radix pass (5- and
+// 10-smooth radices), which a Bluestein length search weighs like it.
+bool stage_kernel_fast(int id);
 
 // Register-resident two-stage super-stages (colpass.hip): the instance for
 // super-radix r with `units` per workgroup (0: the default one), -1 if none;

@@ -114,10 +114,13 @@ bool radix_factor(int64_t n, std::vector<int> &out) {
   return true;
 }
 
-bool passes_specialised(const std::vector<int64_t> &passes) {
+bool passes_specialised(const std::vector<int64_t> &passes, bool fast_only) {
   std::vector<int> rad;
-  for (int64_t r : passes)
-    if (!radix_factor(r, rad) || stage_kernel_id(rad) < 0) return false;
+  for (int64_t r : passes) {
+    if (!radix_factor(r, rad)) return false;
+    const int id = stage_kernel_id(rad);
+    if (id < 0 || (fast_only && !stage_kernel_fast(id))) return false;
+  }
   return true;
 }
 
@@ -179,13 +182,16 @@ int column_segment_bytes(const std::vector<int> &rad, Prec p) {
 // row segments of 128 bytes or more, 3.0 to 4.2 with 64;
 // LDS super-stages at 2.2 to 3.5 by the row-segment bytes their tile reads;
 // the run-time-radix pass at about 1.1.
-double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o) {
+// tcnt: units per sequence (n / r); a register-resident instance takes the
+// pass only when its units per workgroup divide it (one tile, one sequence).
+double pass_time(int64_t r, Prec p, bool first, const PlanOptions &o, int64_t tcnt) {
   std::vector<int> rad;
   if (o.generic_kernels) {
     // persistent-queue plans: register-resident queue instances, else the run-time-radix pass
     return (o.queue_form && col_queue_kernel_id(r, p == Prec::F64) >= 0) ? 1.0 / 4.0 : 1.0 / 1.1;
   }
-  if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0) {
+  if (o.stage_form != 2 && col_kernel_id(r, p == Prec::F64, 0) >= 0 &&
+      col_kernel_id_dividing(r, p == Prec::F64, tcnt) >= 0) {
     // 64-byte segments (the 4096-point instance: 8 units of 32 KiB in one
     // workgroup) run at 3.0 / 4.2
     if (col_run_bytes(r, p == Prec::F64, first) < 128) return first ? 1.0 / 3.0 : 1.0 / 4.2;
@@ -212,7 +218,7 @@ bool cheapest_split(int64_t n, Prec p, const PlanOptions &o, std::vector<int64_t
     enum_splits(n, k, cap, cap, cur, all);
     for (const auto &v : all) {
       double tt = 0.0;
-      for (size_t i = 0; i < v.size(); ++i) tt += pass_time(v[i], p, i == 0, o);
+      for (size_t i = 0; i < v.size(); ++i) tt += pass_time(v[i], p, i == 0, o, n / v[i]);
       const bool better = !have || tt < best_t * 0.98 ||
                           (tt <= best_t * 1.02 && v.size() == best.size() &&
                            *std::min_element(v.begin(), v.end()) > *std::min_element(best.begin(), best.end()));
@@ -309,8 +315,9 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
   // Bluestein over a 13-smooth convolution length M >= 2n-1.  A length that
   // fits one LDS pass is taken as soon as it is found (the whole convolution
   // is one launch).  Otherwise every M up to twice the minimum is weighed by
-  // its HBM sweeps: M x passes, x 2.5 when a pass has no specialised kernel
-  // (the run-time-radix pass runs about that much slower).
+  // its HBM sweeps: M x passes, x 2.5 when a pass has no fast specialised
+  // kernel (the run-time-radix pass and the 5- / 10-smooth instances run
+  // about that much slower than the power-of-two passes).
   Core best;
   double best_cost = 0.0;
   bool have = false;
@@ -354,7 +361,7 @@ Core plan_core(int64_t n, Prec p, const PlanOptions &o) {
       break;
     }
     const double cost = static_cast<double>(M) * static_cast<double>(sub.passes.size()) / 2.0 *
-                        (passes_specialised(sub.passes) ? 1.0 : 2.5);
+                        (passes_specialised(sub.passes, true) ? 1.0 : 2.5);
     if (!have || cost < best_cost) {
       best = sub;
       best_cost = cost;
@@ -468,9 +475,11 @@ void widen_columns(Plan1D &cols, Prec prec, const PlanOptions &o, int64_t ncol, 
     const int64_t rounds = (tiles + kCus - 1) / kCus;
     if ((nseq == 1 || ncol % c == 0) && 4 * tiles >= 3 * rounds * kCus) return;
   }
+  // (no LDS super-stage instance for the whole column either: the
+  // run-time-radix pass would read it one element per column)
   const int seg = column_segment_bytes(cols.core.radices, prec);
   Core mc;
-  if (seg == 0 || seg >= 64 || !split_passes(cols.n, 2, pass_max(o, prec), mc.passes)) return;
+  if (seg >= 64 || !split_passes(cols.n, 2, pass_max(o, prec), mc.passes)) return;
   for (int64_t r : mc.passes) {
     std::vector<int> rad;
     if (!radix_factor(r, rad) || column_segment_bytes(rad, prec) < 64) return;

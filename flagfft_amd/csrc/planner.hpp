@@ -111,8 +111,11 @@ int col_kernel_id(int64_t r, bool dbl, int units);
 int col_queue_kernel_id(int64_t r, bool dbl);
 int col_run_bytes(int64_t r, bool dbl, bool first);
 int col_pair_kernel_id(int64_t r, bool dbl);
-// True when every super-radix of the split has a specialised kernel.
-bool passes_specialised(const std::vector<int64_t> &passes);
+int col_kernel_id_dividing(int64_t r, bool dbl, int64_t tcnt);
+// True when every super-radix of the split has a specialised kernel (fast
+// only: one that runs near the power-of-two passes' rate).
+bool passes_specialised(const std::vector<int64_t> &passes, bool fast_only = false);
+bool stage_kernel_fast(int id);
 
 Core plan_core(int64_t n, Prec p, const PlanOptions &o);
 PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o);

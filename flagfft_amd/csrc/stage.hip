@@ -164,6 +164,7 @@ struct StageEntry {
   int rad[3];
   int c32, e32, c64, e64;
   const void *f[2][2];  // [FP64][StageMode]
+  bool fast;            // runs near the other specialised passes' rate (see stage_kernel_fast)
 };
 
 #define FLAGFFT_STAGE_FNS(C32, E32, C64, E64, ...)                                                     \
@@ -176,9 +177,13 @@ struct StageEntry {
     }                                                                                                  \
   }
 #define FLAGFFT_STAGE2(C32, E32, C64, E64, R0, R1) \
-  {2, {R0, R1, 0}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1)}
+  {2, {R0, R1, 0}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1), true}
 #define FLAGFFT_STAGE3(C32, E32, C64, E64, R0, R1, R2) \
-  {3, {R0, R1, R2}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1, R2)}
+  {3, {R0, R1, R2}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1, R2), true}
+#define FLAGFFT_STAGE2_SLOW(C32, E32, C64, E64, R0, R1) \
+  {2, {R0, R1, 0}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1), false}
+#define FLAGFFT_STAGE3_SLOW(C32, E32, C64, E64, R0, R1, R2) \
+  {3, {R0, R1, R2}, C32, E32, C64, E64, FLAGFFT_STAGE_FNS(C32, E32, C64, E64, R0, R1, R2), false}
 
 const StageEntry kStageTable[] = {
     FLAGFFT_STAGE2(16, 16, 8, 16, 16, 4),       // 64
@@ -194,6 +199,13 @@ const StageEntry kStageTable[] = {
     FLAGFFT_STAGE2(32, 16, 16, 16, 9, 9),       // 81
     FLAGFFT_STAGE2(16, 16, 8, 16, 15, 9),       // 135
     FLAGFFT_STAGE3(8, 16, 4, 16, 9, 9, 9),      // 729
+    // 5- and 10-smooth passes (10^k, 5^k lengths): the run-time-radix pass
+    // reads them at 0.4-0.9 TB/s, these at 0.9-1.9 (MI355X, 8-32 MB
+    // transforms), well below the power-of-two passes
+    FLAGFFT_STAGE2_SLOW(64, 16, 32, 16, 5, 5),       // 25
+    FLAGFFT_STAGE2_SLOW(32, 16, 16, 16, 10, 10),     // 100
+    FLAGFFT_STAGE3_SLOW(32, 16, 16, 16, 5, 5, 5),    // 125
+    FLAGFFT_STAGE3_SLOW(16, 32, 8, 16, 10, 10, 10),  // 1000
     // wider tiles (longer contiguous runs per strided read), picked by the
     // planner's units-per-workgroup choice
     FLAGFFT_STAGE2(64, 16, 32, 16, 16, 8),      // 128
@@ -222,6 +234,8 @@ int stage_kernel_id(const std::vector<int> &radices, bool dbl, int units) {
 }
 
 int stage_kernel_id(const std::vector<int> &radices) { return stage_kernel_id(radices, false, 0); }
+
+bool stage_kernel_fast(int id) { return kStageTable[id].fast; }
 
 void stage_kernel_shape(int id, bool dbl, int &units_per_group, int &ept) {
   units_per_group = dbl ? kStageTable[id].c64 : kStageTable[id].c32;

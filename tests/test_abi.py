@@ -261,6 +261,9 @@ def test_planner_choices(untuned):
     for t in (T.R2C, T.C2R):
         d = _describe([4096, 4096], t, rank=2)
         assert "dc_nyquist_pair" in d and "super_radices=[64,64]" not in d, d
+    # 8192-point columns: no single-pass instance with wide row segments
+    for t in (T.C2C, T.R2C, T.Z2Z):
+        assert "super_radices=[128,64]" in _describe([8192, 8192], t, rank=2)
     assert "dc_nyquist_pair" in _describe([4096, 8192], T.R2C, rank=2)
     assert "dc_nyquist_pair" not in _describe([4096, 4094], T.R2C, rank=2)  # 2048 columns: whole rounds
     assert "dc_nyquist_pair" not in _describe([4096, 4096], T.C2C, rank=2)

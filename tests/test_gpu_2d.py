@@ -206,6 +206,41 @@ def test_4096_point_columns(gpu, shape, batch, single):
                    what=f"2D {n0}x{n1} batch {batch}")
 
 
+# 8192-point columns have no single-pass instance that reads wide row
+# segments (the LDS form would read one element per column): two passes of
+# 128 and 64 points.
+@pytest.mark.parametrize("shape,t", [((8192, 100), ff.Type.C2C), ((8192, 256), ff.Type.R2C),
+                                     ((8192, 36), ff.Type.Z2Z), ((8192, 64), ff.Type.D2Z)],
+                         ids=["c2c-8192x100", "r2c-8192x256", "z2z-8192x36", "d2z-8192x64"])
+def test_8192_point_columns(gpu, shape, t):
+    import torch
+
+    n0, n1 = shape
+    real = t in (ff.Type.R2C, ff.Type.D2Z)
+    dbl = t in (ff.Type.Z2Z, ff.Type.D2Z)
+    h = n1 // 2 + 1 if real else n1
+    if real:
+        x = real_input(n0 + n1, (1, n0, n1), np.float64 if dbl else np.float32)
+        ref = ora.rfft2(x.astype(np.float64))
+    else:
+        x = complex_input(n0 + n1, (1, n0, n1), np.complex128 if dbl else np.complex64)
+        ref = ora.fft2(x.astype(np.complex128))
+    xd = dev_tensor(x, gpu)
+    yd = torch.zeros((1, n0, h), dtype=torch.complex128 if dbl else torch.complex64, device=gpu)
+    with ff.Plan.two_d(n0, n1, t) as p:
+        desc = p.description()
+        assert "super_radices=[128,64]" in desc, desc
+        p.set_stream(stream_of(gpu))
+        {ff.Type.C2C: lambda: p.c2c(xd.data_ptr(), yd.data_ptr(), ff.FORWARD),
+         ff.Type.Z2Z: lambda: p.z2z(xd.data_ptr(), yd.data_ptr(), ff.FORWARD),
+         ff.Type.R2C: lambda: p.r2c(xd.data_ptr(), yd.data_ptr()),
+         ff.Type.D2Z: lambda: p.d2z(xd.data_ptr(), yd.data_ptr())}[t]()
+        sync(gpu)
+    n = n0 * n1
+    check_accuracy(host(yd).reshape(1, -1), ref.reshape(1, -1), t, n, row_len=n0 * h, factor=2.0,
+                   what=f"2D {n0}x{n1}")
+
+
 # One real transform's n1 / 2 + 1 half-spectrum columns of 4096 points would
 # leave one 8-column tile alone on the GPU: the DC and Nyquist columns (real on
 # the real side) share unit 0 of a single register-resident pass instead

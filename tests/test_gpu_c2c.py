@@ -150,3 +150,22 @@ def test_radix_17_and_19_real_roundtrip(gpu, n):
             sync(gpu)
         check_accuracy(host(yd), ora.rfft(x.astype(np.float64)), fwd, n, row_len=n // 2 + 1, what="r2c")
         check_accuracy(host(zd), n * x.astype(np.float64), inv, n, row_len=n, factor=2.0, what="roundtrip")
+
+
+# 5- and 10-smooth lengths run as chains of specialised LDS passes of 25,
+# 100, 125 and 1000 points (k_stage [5x5], [10x10], [5x5x5], [10x10x10]).
+# (400 = [10, 10, 4] and 120 = [15, 8] passes have no specialised instance)
+SMOOTH5 = [(10000, 3, "[10x10]", True), (78125, 2, "[5x5]", True), (100000, 1, "[10x10x10]", True),
+           (15625, 4, "[5x5x5]", True), (1000000, 1, "[10x10x10]", True), (250000, 1, "[5x5]", True),
+           (50000, 2, "[5x5x5]", False), (12000, 2, "[10x10]", False)]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n,batch,inst,all_specialised", SMOOTH5, ids=[f"{c[0]}" for c in SMOOTH5])
+def test_five_smooth_sizes(gpu, n, batch, inst, all_specialised, dtype):
+    t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
+    with ff.Plan.one_d(n, t, batch) as p:
+        desc = p.description()
+    assert "k_stage<" in desc and inst in desc and ("k_pass" not in desc) == all_specialised, desc
+    for direction in (ff.FORWARD, ff.INVERSE):
+        _case(gpu, n, batch, dtype, direction)
