@@ -515,7 +515,7 @@ PlanTree plan_tree(const PlanDesc &d, const PlanOptions &o) {
     // one real transform whose n1 / 2 + 1 half-spectrum columns take two
     // passes only because the last tile would run alone: the DC and Nyquist
     // columns (both real on the real side) share a unit of the single pass
-    // (4096^2 R2C: one 33 us column pass instead of two of 25 us)
+    // (4096^2 R2C: one 41-43 us column pass instead of two of 24-25 us)
     if (half && d.batch == 1 && d.n1 % 2 == 0 && !o.generic_kernels && t.rows.real_form == RealForm::EvenFused &&
         leaf_kernel_id(t.rows.core.radices) >= 0 && t.cols.core.kind == CoreKind::Leaf &&
         col_pair_kernel_id(d.n0, d.prec == Prec::F64) >= 0) {
