@@ -261,6 +261,14 @@ def test_planner_choices(untuned):
     for t in (T.R2C, T.C2R):
         d = _describe([4096, 4096], t, rank=2)
         assert "dc_nyquist_pair" in d and "super_radices=[64,64]" not in d, d
+    # 10^k and 5^k lengths: chains of the 25 / 100 / 125 / 1000-point LDS
+    # passes; a Bluestein convolution keeps a power-of-two length (those
+    # passes run well below the power-of-two ones)
+    assert "super_radices=[100,100]" in _describe(10000, batch=100)
+    assert "super_radices=[1000,1000]" in _describe(1000000, batch=4)
+    assert "super_radices=[125,25,25]" in _describe(78125, batch=16)
+    assert "conv_length=1048576" in _describe(419017, batch=64)
+    assert "conv_length=262144" in _describe(116724, batch=96)
     # 8192-point columns: no single-pass instance with wide row segments
     for t in (T.C2C, T.R2C, T.Z2Z):
         assert "super_radices=[128,64]" in _describe([8192, 8192], t, rank=2)
