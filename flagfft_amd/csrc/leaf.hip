@@ -246,8 +246,10 @@ const LeafEntry kLeafTable[] = {
     FLAGFFT_LEAF4(16, 16, 15, 10, 10, 2),
     FLAGFFT_LEAF3(16, 16, 16, 10, 10),
     FLAGFFT_LEAF3(16, 16, 9, 9, 9),
+    FLAGFFT_LEAF4(16, 16, 15, 10, 10, 4),  // 6000 (and the R2C / C2R halves of 12000)
     // whole-sequence-in-LDS lengths above the default single-pass limit
-    FLAGFFT_LEAF4(22, 22, 15, 9, 9, 9),  // 10935 = 3^7 * 5
+    FLAGFFT_LEAF4(22, 22, 15, 9, 9, 9),    // 10935 = 3^7 * 5
+    FLAGFFT_LEAF4(20, 20, 10, 10, 10, 10),  // 10000 (FP32: 80 KB of LDS; FP64 runs two passes)
 };
 
 }  // namespace

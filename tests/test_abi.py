@@ -264,7 +264,8 @@ def test_planner_choices(untuned):
     # 10^k and 5^k lengths: chains of the 25 / 100 / 125 / 1000-point LDS
     # passes; a Bluestein convolution keeps a power-of-two length (those
     # passes run well below the power-of-two ones)
-    assert "super_radices=[100,100]" in _describe(10000, batch=100)
+    assert "Leaf(n=10000, radices=[10,10,10,10]" in _describe(10000, batch=100)  # FP32: 80 KB in LDS
+    assert "super_radices=[100,100]" in _describe(10000, T.Z2Z, batch=100)
     assert "super_radices=[1000,1000]" in _describe(1000000, batch=4)
     assert "super_radices=[125,25,25]" in _describe(78125, batch=16)
     assert "conv_length=1048576" in _describe(419017, batch=64)

@@ -154,18 +154,23 @@ def test_radix_17_and_19_real_roundtrip(gpu, n):
 
 # 5- and 10-smooth lengths run as chains of specialised LDS passes of 25,
 # 100, 125 and 1000 points (k_stage [5x5], [10x10], [5x5x5], [10x10x10]).
-# (400 = [10, 10, 4] and 120 = [15, 8] passes have no specialised instance)
-SMOOTH5 = [(10000, 3, "[10x10]", True), (78125, 2, "[5x5]", True), (100000, 1, "[10x10x10]", True),
-           (15625, 4, "[5x5x5]", True), (1000000, 1, "[10x10x10]", True), (250000, 1, "[5x5]", True),
-           (50000, 2, "[5x5x5]", False), (12000, 2, "[10x10]", False)]
+# (400 = [10, 10, 4] and 120 = [15, 8] passes have no specialised instance;
+# 10000 FP32 and 6000 are single-pass leaves with the sequence in LDS)
+SMOOTH5 = [(10000, 3, "k_leaf<fp32,20,0>[10x10x10x10]", "[10x10]", True), (78125, 2, "[5x5]", "[5x5]", True),
+           (100000, 1, "[10x10x10]", "[10x10x10]", True), (15625, 4, "[5x5x5]", "[5x5x5]", True),
+           (1000000, 1, "[10x10x10]", "[10x10x10]", True), (250000, 1, "[5x5]", "[5x5]", True),
+           (50000, 2, "[5x5x5]", "[5x5x5]", False), (12000, 2, "[10x10]", "[10x10]", False),
+           (6000, 4, "k_leaf<fp32,16,0>[15x10x10x4]", "k_leaf<fp64,16,0>[15x10x10x4]", True)]
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
-@pytest.mark.parametrize("n,batch,inst,all_specialised", SMOOTH5, ids=[f"{c[0]}" for c in SMOOTH5])
-def test_five_smooth_sizes(gpu, n, batch, inst, all_specialised, dtype):
-    t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
+@pytest.mark.parametrize("n,batch,inst32,inst64,all_specialised", SMOOTH5, ids=[f"{c[0]}" for c in SMOOTH5])
+def test_five_smooth_sizes(gpu, n, batch, inst32, inst64, all_specialised, dtype):
+    dbl = dtype == np.complex128
+    t = ff.Type.Z2Z if dbl else ff.Type.C2C
     with ff.Plan.one_d(n, t, batch) as p:
         desc = p.description()
-    assert "k_stage<" in desc and inst in desc and ("k_pass" not in desc) == all_specialised, desc
+    inst = inst64 if dbl else inst32
+    assert inst in desc and ("k_pass" not in desc) == all_specialised, desc
     for direction in (ff.FORWARD, ff.INVERSE):
         _case(gpu, n, batch, dtype, direction)
