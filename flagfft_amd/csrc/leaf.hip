@@ -247,6 +247,25 @@ const LeafEntry kLeafTable[] = {
     FLAGFFT_LEAF3(16, 16, 16, 10, 10),
     FLAGFFT_LEAF3(16, 16, 9, 9, 9),
     FLAGFFT_LEAF4(16, 16, 15, 10, 10, 4),  // 6000 (and the R2C / C2R halves of 12000)
+    // 3, 5 and 15 times a power of two (the run-time-radix single pass is
+    // about 3x slower)
+    FLAGFFT_LEAF3(16, 16, 16, 12, 2),       // 384
+    FLAGFFT_LEAF3(16, 16, 16, 12, 4),       // 768
+    FLAGFFT_LEAF3(16, 16, 16, 12, 8),       // 1536
+    FLAGFFT_LEAF3(16, 16, 16, 16, 12),      // 3072
+    FLAGFFT_LEAF4(16, 16, 16, 16, 12, 2),   // 6144
+    FLAGFFT_LEAF3(16, 16, 16, 10, 4),       // 640
+    FLAGFFT_LEAF3(16, 16, 16, 10, 8),       // 1280
+    FLAGFFT_LEAF3(16, 16, 16, 16, 10),      // 2560
+    FLAGFFT_LEAF4(16, 16, 16, 16, 10, 2),   // 5120
+    FLAGFFT_LEAF3(16, 16, 16, 15, 4),       // 960
+    FLAGFFT_LEAF3(16, 16, 16, 15, 8),       // 1920
+    FLAGFFT_LEAF3(16, 16, 16, 16, 15),      // 3840
+    FLAGFFT_LEAF3(16, 16, 15, 10, 8),       // 1200
+    FLAGFFT_LEAF3(16, 16, 16, 15, 10),      // 2400
+    FLAGFFT_LEAF4(16, 16, 16, 15, 10, 2),   // 4800
+    FLAGFFT_LEAF4(16, 16, 16, 10, 10, 2),   // 3200
+    FLAGFFT_LEAF4(16, 16, 16, 10, 10, 4),   // 6400
     // whole-sequence-in-LDS lengths above the default single-pass limit
     FLAGFFT_LEAF4(22, 22, 15, 9, 9, 9),    // 10935 = 3^7 * 5
     FLAGFFT_LEAF4(20, 20, 10, 10, 10, 10),  // 10000 (FP32: 80 KB of LDS; FP64 runs two passes)

@@ -174,3 +174,20 @@ def test_five_smooth_sizes(gpu, n, batch, inst32, inst64, all_specialised, dtype
     assert inst in desc and ("k_pass" not in desc) == all_specialised, desc
     for direction in (ff.FORWARD, ff.INVERSE):
         _case(gpu, n, batch, dtype, direction)
+
+
+# 3, 5 and 15 times a power of two: specialised single-pass leaves (FP64
+# lengths past 4096 whose sequence fits LDS as well).
+SMOOTH_LEAVES = [384, 768, 1536, 3072, 6144, 640, 1280, 2560, 5120, 960, 1920, 3840, 1200, 2400, 4800, 3200, 6400]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
+@pytest.mark.parametrize("n", SMOOTH_LEAVES)
+def test_smooth_leaf_sizes(gpu, n, dtype):
+    t = ff.Type.Z2Z if dtype == np.complex128 else ff.Type.C2C
+    with ff.Plan.one_d(n, t, 5) as p:
+        desc = p.description()
+    assert "k_leaf<" in desc and "k_pass" not in desc, desc
+    for batch in (1, 5):
+        for direction in (ff.FORWARD, ff.INVERSE):
+            _case(gpu, n, batch, dtype, direction)

@@ -16,7 +16,9 @@ pytestmark = pytest.mark.gpu
 
 # even lengths whose half is a Rader (194, 2018) or Bluestein (9998, 65538) prime-factor core
 SIZES = [2, 3, 15, 16, 64, 67, 81, 194, 243, 256, 361, 997, 1009, 2018, 4096, 6000, 8192, 9998, 10007, 16384, 32768,
-         65536 + 2]
+         65536 + 2,
+         # halves on the 3 / 5 / 15 x 2^k and 6000 / 10000-point leaves
+         3072, 2560, 3840, 4800, 12000, 20000]
 
 
 def _r2c(dev, x, n, batch, dbl):
