@@ -247,8 +247,8 @@ const LeafEntry kLeafTable[] = {
     FLAGFFT_LEAF3(16, 16, 16, 10, 10),
     FLAGFFT_LEAF3(16, 16, 9, 9, 9),
     FLAGFFT_LEAF4(16, 16, 15, 10, 10, 4),  // 6000 (and the R2C / C2R halves of 12000)
-    // 3, 5 and 15 times a power of two (the run-time-radix single pass is
-    // about 3x slower)
+    // 3, 5, 7, 9 and 15 times a power of two, 1000 times one (the
+    // run-time-radix single pass is about 3x slower)
     FLAGFFT_LEAF3(16, 16, 16, 12, 2),       // 384
     FLAGFFT_LEAF3(16, 16, 16, 12, 4),       // 768
     FLAGFFT_LEAF3(16, 16, 16, 12, 8),       // 1536
@@ -266,6 +266,19 @@ const LeafEntry kLeafTable[] = {
     FLAGFFT_LEAF4(16, 16, 16, 15, 10, 2),   // 4800
     FLAGFFT_LEAF4(16, 16, 16, 10, 10, 2),   // 3200
     FLAGFFT_LEAF4(16, 16, 16, 10, 10, 4),   // 6400
+    FLAGFFT_LEAF3(16, 16, 16, 9, 8),        // 1152
+    FLAGFFT_LEAF3(16, 16, 16, 16, 9),       // 2304
+    FLAGFFT_LEAF4(16, 16, 16, 16, 9, 2),    // 4608
+    FLAGFFT_LEAF3(16, 16, 16, 14, 4),       // 896
+    FLAGFFT_LEAF3(16, 16, 16, 14, 8),       // 1792
+    FLAGFFT_LEAF3(16, 16, 16, 16, 14),      // 3584
+    FLAGFFT_LEAF4(16, 16, 16, 16, 14, 2),   // 7168
+    FLAGFFT_LEAF3(16, 16, 15, 12, 8),       // 1440
+    FLAGFFT_LEAF3(16, 16, 16, 15, 12),      // 2880
+    FLAGFFT_LEAF4(16, 16, 16, 15, 12, 2),   // 5760
+    FLAGFFT_LEAF4(16, 16, 10, 10, 10, 2),   // 2000
+    FLAGFFT_LEAF4(16, 16, 10, 10, 10, 4),   // 4000
+    FLAGFFT_LEAF4(16, 16, 10, 10, 10, 8),   // 8000
     // whole-sequence-in-LDS lengths above the default single-pass limit
     FLAGFFT_LEAF4(22, 22, 15, 9, 9, 9),    // 10935 = 3^7 * 5
     FLAGFFT_LEAF4(20, 20, 10, 10, 10, 10),  // 10000 (FP32: 80 KB of LDS; FP64 runs two passes)

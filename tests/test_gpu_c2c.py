@@ -176,9 +176,10 @@ def test_five_smooth_sizes(gpu, n, batch, inst32, inst64, all_specialised, dtype
         _case(gpu, n, batch, dtype, direction)
 
 
-# 3, 5 and 15 times a power of two: specialised single-pass leaves (FP64
-# lengths past 4096 whose sequence fits LDS as well).
-SMOOTH_LEAVES = [384, 768, 1536, 3072, 6144, 640, 1280, 2560, 5120, 960, 1920, 3840, 1200, 2400, 4800, 3200, 6400]
+# 3, 5, 7, 9 and 15 times a power of two and 1000 times one: specialised
+# single-pass leaves (FP64 lengths past 4096 whose sequence fits LDS as well).
+SMOOTH_LEAVES = [384, 768, 1536, 3072, 6144, 640, 1280, 2560, 5120, 960, 1920, 3840, 1200, 2400, 4800, 3200, 6400,
+                 1152, 2304, 4608, 896, 1792, 3584, 7168, 1440, 2880, 5760, 2000, 4000, 8000]
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128], ids=["c2c", "z2z"])
