@@ -745,24 +745,44 @@ __global__ __launch_bounds__(kQueueThreads) __attribute__((amdgpu_waves_per_eu(W
 // Split-spectrum passes for even-length real transforms that run multi-pass:
 //   post: Z (h complex, contiguous) -> X (h+1 bins);  pre: X -> conj(Z').
 // ---------------------------------------------------------------------------
-// Tile blk of nblk (256 threads each) strides over the elements.
+// Item g of a sequence set with `per` items per sequence: sequence q and
+// item k (32-bit division when the count allows).
+__device__ __forceinline__ void real_item(int64_t g, int64_t total, int per, int64_t &q, int &k) {
+  if (total < (int64_t{1} << 31)) {
+    const uint32_t gg = static_cast<uint32_t>(g), qq = gg / static_cast<uint32_t>(per);
+    q = qq;
+    k = static_cast<int>(gg - qq * static_cast<uint32_t>(per));
+  } else {
+    q = g / per;
+    k = static_cast<int>(g - q * per);
+  }
+}
+
+// Tile blk of nblk (256 threads each) strides over the items: item k <= h/2
+// of a sequence reads Z_k and Z_{h-k} once and writes both X_k and X_{h-k}
+// (X_k = E_k + w^k O_k, X_{h-k} = conj(E_k - w^k O_k), as the fused leaf).
 template <typename T>
 __device__ __forceinline__ void real_post_body(const RealPassArgs &a, int blk, int nblk) {
   const int h = a.n / 2;
-  const int64_t total = a.nseqs * (h + 1);
+  const int per = h / 2 + 1;
+  const int64_t total = a.nseqs * per;
   const GlbC<T> *wn = gc<T>(a.tw_half);
   const T half = T(0.5);
   for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
        g += static_cast<int64_t>(nblk) * kQueueThreads) {
-    const int64_t q = g / (h + 1);
-    const int k = static_cast<int>(g - q * (h + 1));
-    const int64_t b = q / a.ncol, c = q - b * a.ncol;
+    int64_t q;
+    int k;
+    real_item(g, total, per, q, k);
+    const int64_t b = a.ncol == 1 ? q : q / a.ncol, c = q - b * a.ncol;
     const GlbC<T> *Z = gc<T>(a.in) + b * a.gi.bs + c * a.gi.cs;
-    const V2<T> zk = ldg(Z + (k == h ? 0 : k) * a.gi.es);
+    GlbC<T> *X = gc<T>(a.out) + b * a.go.bs + c * a.go.cs;
+    const V2<T> zk = ldg(Z + k * a.gi.es);
     const V2<T> zc = cconj(ldg(Z + (k == 0 ? 0 : h - k) * a.gi.es));
     const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
     const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
-    stg(gc<T>(a.out) + b * a.go.bs + c * a.go.cs + k * a.go.es, cadd(ev, cmul(ldg(wn + k), od)));
+    const V2<T> wo = cmul(ldg(wn + k), od);
+    stg(X + k * a.go.es, cadd(ev, wo));
+    if (2 * k != h) stg(X + (h - k) * a.go.es, cconj(csub(ev, wo)));
   }
 }
 
@@ -771,26 +791,39 @@ __global__ __launch_bounds__(kQueueThreads) void k_real_post(const RealPassArgs 
   real_post_body<T>(a, blockIdx.x, gridDim.x);
 }
 
+// Item k <= h/2 reads X_k and X_{h-k} once and writes Z'_k and (k > 0)
+// Z'_{h-k}: the roles of the two bins swap and conj(w^(h-k)) = -w^k.
 template <typename T>
 __device__ __forceinline__ void real_pre_body(const RealPassArgs &a, int blk, int nblk) {
   const int h = a.n / 2;
-  const int64_t total = a.nseqs * h;
+  const int per = h / 2 + 1;
+  const int64_t total = a.nseqs * per;
   const GlbC<T> *wn = gc<T>(a.tw_half);
   for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
        g += static_cast<int64_t>(nblk) * kQueueThreads) {
-    const int64_t q = g / h;
-    const int k = static_cast<int>(g - q * h);
-    const int64_t b = q / a.ncol, c = q - b * a.ncol;
+    int64_t q;
+    int k;
+    real_item(g, total, per, q, k);
+    const int64_t b = a.ncol == 1 ? q : q / a.ncol, c = q - b * a.ncol;
     const GlbC<T> *Xs = gc<T>(a.in) + b * a.gi.bs + c * a.gi.cs;
+    GlbC<T> *Zo = gc<T>(a.out) + b * a.go.bs + c * a.go.cs;
     V2<T> xk = ldg(Xs + k * a.gi.es);
     V2<T> xh = ldg(Xs + (h - k) * a.gi.es);
     if (k == 0) {
       xk.y = T(0);
       xh.y = T(0);
     }
-    xh.y = -xh.y;
-    const V2<T> t = cmul_pi(cmul(csub(xk, xh), cconj(ldg(wn + k))));
-    stg(gc<T>(a.out) + b * a.go.bs + c * a.go.cs + k * a.go.es, cconj(cadd(cadd(xk, xh), t)));
+    const V2<T> w = ldg(wn + k);
+    {
+      const V2<T> cx = cconj(xh);
+      const V2<T> t = cmul_pi(cmul(csub(xk, cx), cconj(w)));
+      stg(Zo + k * a.go.es, cconj(cadd(cadd(xk, cx), t)));
+    }
+    if (k != 0 && 2 * k != h) {
+      const V2<T> cx = cconj(xk);
+      const V2<T> t = cmul_pi(cmul(csub(xh, cx), V2<T>{-w.x, -w.y}));
+      stg(Zo + (h - k) * a.go.es, cconj(cadd(cadd(xh, cx), t)));
+    }
   }
 }
 
@@ -947,8 +980,9 @@ const void *real_pre_kernel(bool dbl) {
   return dbl ? reinterpret_cast<const void *>(dev::k_real_pre<double>)
              : reinterpret_cast<const void *>(dev::k_real_pre<float>);
 }
-LaunchShape real_post_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 2 + 1), 256), dim3(256), 0}; }
-LaunchShape real_pre_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 2), 256), dim3(256), 0}; }
+// (one item per pair of bins k, h - k: h / 2 + 1 per sequence, h = n / 2)
+LaunchShape real_post_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 4 + 1), 256), dim3(256), 0}; }
+LaunchShape real_pre_shape(const RealPassArgs &a) { return LaunchShape{grid_for(a.nseqs * (a.n / 4 + 1), 256), dim3(256), 0}; }
 
 hipError_t gen_roots(void *out, bool dbl, int64_t count, int64_t n, int64_t mult, hipStream_t st) {
   dim3 g = grid_for(count, 256);
