@@ -1607,7 +1607,7 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
   // EvenMulti: n/2-point core (multi-pass chain, Rader or Bluestein) plus a split-spectrum pass
   const Geometry gws{1, h, h * s.ncol};
   const size_t wbytes = static_cast<size_t>(s.nbatch * s.ncol * h) * eb();
-  const void *tw_half = roots(h + 1, n, 1);
+  const TwoLevel tw = two_level(n);
   if (pl.real_fwd) {
     const int W = pick_ws({src, dst}, wbytes);
     SeqSet sc = s;
@@ -1625,7 +1625,9 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     st.ra.nseqs = s.nbatch * s.ncol;
     st.ra.ncol = static_cast<int32_t>(s.ncol);
     st.ra.n = static_cast<int32_t>(n);
-    st.ra.tw_half = tw_half;
+    st.ra.tw_lo = tw.lo;
+    st.ra.tw_hi = tw.hi;
+    st.ra.tw_bits = tw.bits;
     st.bytes_moved = st.ra.nseqs * (h + (h + 1)) * eb();
     st.label = "k_real_post n=" + std::to_string(n);
     ex_.steps.push_back(st);
@@ -1641,7 +1643,9 @@ void Builder::emit_plan1d(const Plan1D &pl, const SeqSet &s, int src, int dst, b
     st.ra.nseqs = s.nbatch * s.ncol;
     st.ra.ncol = static_cast<int32_t>(s.ncol);
     st.ra.n = static_cast<int32_t>(n);
-    st.ra.tw_half = tw_half;
+    st.ra.tw_lo = tw.lo;
+    st.ra.tw_hi = tw.hi;
+    st.ra.tw_bits = tw.bits;
     st.bytes_moved = st.ra.nseqs * ((h + 1) + h) * eb();
     st.label = "k_real_pre n=" + std::to_string(n);
     ex_.steps.push_back(st);

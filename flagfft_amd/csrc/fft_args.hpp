@@ -239,7 +239,10 @@ struct RealPassArgs {
   int64_t nseqs;
   int32_t ncol;
   int32_t n;            // real length (even)
-  const void *tw_half;  // w_n^k, k <= n/2
+  // w_n^k = tw_hi[k >> tw_bits] tw_lo[k & (2^tw_bits - 1)] (two small tables
+  // that stay in cache instead of one of n/2 + 1 entries read beside the data)
+  const void *tw_lo, *tw_hi;
+  int32_t tw_bits;
   int32_t conj_flag;
 };
 

@@ -766,7 +766,9 @@ __device__ __forceinline__ void real_post_body(const RealPassArgs &a, int blk, i
   const int h = a.n / 2;
   const int per = h / 2 + 1;
   const int64_t total = a.nseqs * per;
-  const GlbC<T> *wn = gc<T>(a.tw_half);
+  const GlbC<T> *tw_lo = gc<T>(a.tw_lo);
+  const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
+  const int mask = (1 << a.tw_bits) - 1;
   const T half = T(0.5);
   for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
        g += static_cast<int64_t>(nblk) * kQueueThreads) {
@@ -780,7 +782,7 @@ __device__ __forceinline__ void real_post_body(const RealPassArgs &a, int blk, i
     const V2<T> zc = cconj(ldg(Z + (k == 0 ? 0 : h - k) * a.gi.es));
     const V2<T> ev{(zk.x + zc.x) * half, (zk.y + zc.y) * half};
     const V2<T> od = cmul_mi(V2<T>{(zk.x - zc.x) * half, (zk.y - zc.y) * half});
-    const V2<T> wo = cmul(ldg(wn + k), od);
+    const V2<T> wo = cmul(cmul(ldg(tw_hi + (k >> a.tw_bits)), ldg(tw_lo + (k & mask))), od);
     stg(X + k * a.go.es, cadd(ev, wo));
     if (2 * k != h) stg(X + (h - k) * a.go.es, cconj(csub(ev, wo)));
   }
@@ -798,7 +800,9 @@ __device__ __forceinline__ void real_pre_body(const RealPassArgs &a, int blk, in
   const int h = a.n / 2;
   const int per = h / 2 + 1;
   const int64_t total = a.nseqs * per;
-  const GlbC<T> *wn = gc<T>(a.tw_half);
+  const GlbC<T> *tw_lo = gc<T>(a.tw_lo);
+  const GlbC<T> *tw_hi = gc<T>(a.tw_hi);
+  const int mask = (1 << a.tw_bits) - 1;
   for (int64_t g = blk * static_cast<int64_t>(kQueueThreads) + threadIdx.x; g < total;
        g += static_cast<int64_t>(nblk) * kQueueThreads) {
     int64_t q;
@@ -813,7 +817,7 @@ __device__ __forceinline__ void real_pre_body(const RealPassArgs &a, int blk, in
       xk.y = T(0);
       xh.y = T(0);
     }
-    const V2<T> w = ldg(wn + k);
+    const V2<T> w = cmul(ldg(tw_hi + (k >> a.tw_bits)), ldg(tw_lo + (k & mask)));
     {
       const V2<T> cx = cconj(xh);
       const V2<T> t = cmul_pi(cmul(csub(xk, cx), cconj(w)));
